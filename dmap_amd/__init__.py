@@ -1,0 +1,280 @@
+"""dmap_amd: ray / displaced-surface intersection on AMD MI355X (gfx950).
+
+Python view of libdmap_amd.so (C ABI: include/dmap_amd.h), mirroring the
+reference's intersector interface (code/cpp/src/):
+
+  DisplacementTile      DisplacementAsset + TaylorPyramid
+                        (displacement_asset.cpp:16-35, taylor_pyramid.cpp:215-251)
+  ObjectIntersector     ObjectIntersector (displaced_intersector.h:176-198);
+                        .intersect = closest hit (intersect_flat :578-599),
+                        .occluded = any hit (displaced_geometry.cpp:79-106)
+  TraversalOptions      TraversalOptions (traversal_options.h:30-38)
+
+Device memory and streams are torch's (ROCm); every computation runs in the
+HIP kernels of the library. There is no CPU fallback: without the built
+library, or without a GPU, the calls raise.
+"""
+import ctypes
+from dataclasses import dataclass
+from pathlib import Path
+
+import numpy as np
+
+_HERE = Path(__file__).resolve().parent
+LIB_PATH = _HERE / "_lib" / "libdmap_amd.so"
+
+DMAP_OK = 0
+BOUND = {"box": 0, "slab": 1, "both": 2}
+LEAF = {"newton": 0, "two_triangle": 1}
+PRECISION = {"fp32": 0, "fp64": 1}
+BUILD = {"fold": 0, "direct": 1}
+CHANNELS = ("h0", "gu", "gv", "r", "ru", "rv", "h_min", "h_max")
+STAT_NAMES = ("triangle_tests", "node_tests", "leaf_tests", "sub_squares", "newton_runs", "newton_iters")
+
+# Exported symbols of include/dmap_amd.h (checked by the CPU tests).
+EXPORTS = ("dmap_last_error", "dmap_version", "dmap_ctx_create", "dmap_ctx_destroy", "dmap_tile_create",
+           "dmap_tile_rebuild", "dmap_tile_info", "dmap_tile_read_level", "dmap_tile_read_nodes",
+           "dmap_tile_destroy", "dmap_object_create", "dmap_object_triangle_info", "dmap_object_destroy",
+           "dmap_intersect_closest", "dmap_intersect_any")
+
+
+class DmapError(RuntimeError):
+    pass
+
+
+class _Options(ctypes.Structure):
+    _fields_ = [("bound", ctypes.c_int), ("slab_max_level", ctypes.c_int), ("leaf", ctypes.c_int),
+                ("precision", ctypes.c_int)]
+
+
+_FP = ctypes.c_void_p
+
+
+class _Rays(ctypes.Structure):
+    _fields_ = [(n, _FP) for n in ("ox", "oy", "oz", "dx", "dy", "dz", "tmin", "tmax")]
+
+
+class _Hits(ctypes.Structure):
+    _fields_ = [(n, _FP) for n in ("t", "s", "tt", "bu", "bv", "nx", "ny", "nz", "prim", "texel_i", "texel_j")]
+
+
+class _Stats(ctypes.Structure):
+    _fields_ = [(n, _FP) for n in STAT_NAMES]
+
+
+_lib = None
+
+
+def load_library():
+    """The built library; raises when it is missing (no fallback path)."""
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise DmapError(f"{LIB_PATH} is not built: run python -m dmap_amd.build")
+        L = ctypes.CDLL(str(LIB_PATH))
+        c = ctypes
+        vp = c.c_void_p
+        L.dmap_last_error.restype = c.c_char_p
+        L.dmap_version.restype = c.c_char_p
+        L.dmap_ctx_create.argtypes = [c.c_int, c.POINTER(vp)]
+        L.dmap_ctx_destroy.argtypes = [vp]
+        L.dmap_tile_create.argtypes = [vp, vp, c.c_int, c.c_int, c.c_int, c.c_double, c.c_double, c.c_int, vp,
+                                       c.POINTER(vp)]
+        L.dmap_tile_rebuild.argtypes = [vp, vp]
+        L.dmap_tile_info.argtypes = [vp, c.POINTER(c.c_int), c.POINTER(c.c_int), c.POINTER(c.c_int64)]
+        L.dmap_tile_read_level.argtypes = [vp, c.c_int, vp]
+        L.dmap_tile_read_nodes.argtypes = [vp, vp]
+        L.dmap_tile_destroy.argtypes = [vp]
+        L.dmap_object_create.argtypes = [vp, vp, c.c_int, vp, vp, vp, c.POINTER(vp)]
+        L.dmap_object_triangle_info.argtypes = [vp, c.c_int, c.POINTER(c.c_int), c.POINTER(c.c_int), vp, vp, vp]
+        L.dmap_object_destroy.argtypes = [vp]
+        L.dmap_intersect_closest.argtypes = [vp, c.POINTER(_Rays), c.c_int64, c.POINTER(_Hits), c.c_int,
+                                             c.POINTER(_Options), c.POINTER(_Stats), vp]
+        L.dmap_intersect_any.argtypes = [vp, c.POINTER(_Rays), c.c_int64, vp, c.c_int, c.POINTER(_Options), vp]
+        _lib = L
+    return _lib
+
+
+def _check(rc, what):
+    if rc != DMAP_OK:
+        msg = load_library().dmap_last_error().decode()
+        raise DmapError(f"{what} failed ({rc}): {msg}")
+
+
+def _torch():
+    import torch
+    if not torch.cuda.is_available():
+        raise DmapError("dmap_amd needs a ROCm GPU (torch.cuda.is_available() is False)")
+    return torch
+
+
+def _stream_ptr(stream):
+    torch = _torch()
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return ctypes.c_void_p(s.cuda_stream)
+
+
+@dataclass
+class TraversalOptions:
+    """TraversalOptions (traversal_options.h:30-38) plus the device precision."""
+    bound: str = "both"
+    slab_max_level: int = 1
+    leaf: str = "newton"
+    precision: str = "fp32"
+
+    def _c(self):
+        return _Options(BOUND[self.bound], int(self.slab_max_level), LEAF[self.leaf], PRECISION[self.precision])
+
+
+class Context:
+    """One HIP device (dmap_ctx)."""
+
+    def __init__(self, device=0):
+        _torch()
+        self._h = ctypes.c_void_p()
+        _check(load_library().dmap_ctx_create(int(device), ctypes.byref(self._h)), "dmap_ctx_create")
+        self.device = device
+
+    def __del__(self):
+        if getattr(self, "_h", None) and self._h.value and _lib is not None:
+            _lib.dmap_ctx_destroy(self._h)
+            self._h = None
+
+
+_default_ctx = None
+
+
+def default_context():
+    global _default_ctx
+    if _default_ctx is None:
+        _default_ctx = Context(0)
+    return _default_ctx
+
+
+class DisplacementTile:
+    """A resident displacement tile: the closed node grid and its Taylor
+    pyramid, built on the device (h = strength * texel + offset)."""
+
+    def __init__(self, texels, strength=1.0, offset=0.0, repeat=True, build="fold", ctx=None, stream=None):
+        torch = _torch()
+        self.ctx = ctx or default_context()
+        if isinstance(texels, torch.Tensor) and texels.is_cuda:
+            t = texels.to(torch.float32).contiguous()
+            on_device = 1
+            ptr = t.data_ptr()
+        else:
+            t = np.ascontiguousarray(texels, dtype=np.float32)
+            on_device = 0
+            ptr = t.ctypes.data
+        n = t.shape[0]
+        if t.shape != (n, n):
+            raise DmapError("texels must be square")
+        self._h = ctypes.c_void_p()
+        _check(load_library().dmap_tile_create(self.ctx._h, ctypes.c_void_p(ptr), n, on_device, 1 if repeat else 0,
+                                               float(strength), float(offset), BUILD[build], _stream_ptr(stream),
+                                               ctypes.byref(self._h)), "dmap_tile_create")
+        self._keep = t
+        ni, nl, nb = ctypes.c_int(), ctypes.c_int(), ctypes.c_int64()
+        _check(_lib.dmap_tile_info(self._h, ctypes.byref(ni), ctypes.byref(nl), ctypes.byref(nb)), "dmap_tile_info")
+        self.n_leaf, self.n_levels, self.device_bytes = ni.value, nl.value, nb.value
+        self.strength, self.offset, self.repeat, self.build = float(strength), float(offset), bool(repeat), build
+
+    def rebuild(self, stream=None):
+        _check(_lib.dmap_tile_rebuild(self._h, _stream_ptr(stream)), "dmap_tile_rebuild")
+
+    def level(self, k):
+        """Channels of level k as a dict of (m, m) float32 arrays [j, i]."""
+        m = self.n_leaf >> k
+        out = np.empty((8, m, m), dtype=np.float32)
+        _check(_lib.dmap_tile_read_level(self._h, int(k), ctypes.c_void_p(out.ctypes.data)), "dmap_tile_read_level")
+        return {ch: out[q] for q, ch in enumerate(CHANNELS)}
+
+    def nodes(self):
+        n = self.n_leaf + 1
+        out = np.empty((n, n), dtype=np.float32)
+        _check(_lib.dmap_tile_read_nodes(self._h, ctypes.c_void_p(out.ctypes.data)), "dmap_tile_read_nodes")
+        return out
+
+    def __del__(self):
+        if getattr(self, "_h", None) and self._h.value and _lib is not None:
+            _lib.dmap_tile_destroy(self._h)
+            self._h = None
+
+
+class ObjectIntersector:
+    """Base triangles over one tile (ObjectIntersector). triangles: (n, 24)
+    float64 rows p0 p1 p2 (9), vertex normals m0 m1 m2 (9), uvs t0 t1 t2 (6)."""
+
+    def __init__(self, tile, triangles, ctx=None):
+        self.tile = tile
+        self.ctx = ctx or tile.ctx
+        tri = np.ascontiguousarray(triangles, dtype=np.float64).reshape(-1, 24)
+        self.n_tri = tri.shape[0]
+        pos = np.ascontiguousarray(tri[:, 0:9])
+        nrm = np.ascontiguousarray(tri[:, 9:18])
+        uvs = np.ascontiguousarray(tri[:, 18:24])
+        self._h = ctypes.c_void_p()
+        _check(load_library().dmap_object_create(self.ctx._h, tile._h, self.n_tri, ctypes.c_void_p(pos.ctypes.data),
+                                                 ctypes.c_void_p(nrm.ctypes.data), ctypes.c_void_p(uvs.ctypes.data),
+                                                 ctypes.byref(self._h)), "dmap_object_create")
+
+    def triangle_info(self, k):
+        rl, nr = ctypes.c_int(), ctypes.c_int()
+        roots = np.zeros(8, dtype=np.int32)
+        aabb = np.zeros(6, dtype=np.float32)
+        hr = np.zeros(2, dtype=np.float32)
+        _check(_lib.dmap_object_triangle_info(self._h, int(k), ctypes.byref(rl), ctypes.byref(nr),
+                                              ctypes.c_void_p(roots.ctypes.data), ctypes.c_void_p(aabb.ctypes.data),
+                                              ctypes.c_void_p(hr.ctypes.data)), "dmap_object_triangle_info")
+        return {"root_level": rl.value, "roots": roots[:2 * nr.value].reshape(-1, 2), "aabb": aabb.reshape(2, 3),
+                "h_range": hr}
+
+    @staticmethod
+    def _rays(rays):
+        torch = _torch()
+        if not (isinstance(rays, torch.Tensor) and rays.is_cuda and rays.dtype == torch.float32
+                and rays.dim() == 2 and rays.shape[0] == 8 and rays.is_contiguous()):
+            raise DmapError("rays must be a contiguous (8, n) float32 device tensor: o xyz, d xyz, tmin, tmax")
+        p = [rays[q].data_ptr() for q in range(8)]
+        return _Rays(*p), rays.shape[1]
+
+    def alloc_hits(self, n):
+        torch = _torch()
+        return {"f": torch.empty((8, n), dtype=torch.float32, device="cuda"),
+                "i": torch.empty((3, n), dtype=torch.int32, device="cuda")}
+
+    def intersect(self, rays, options=None, prim_mask=-1, stats=False, out=None, stream=None):
+        """Closest hit of each ray; returns device tensors t, st, bary, ng,
+        prim, texel (and the per-ray work counters when stats is set)."""
+        torch = _torch()
+        r, n = self._rays(rays)
+        hits = out or self.alloc_hits(n)
+        F, I = hits["f"], hits["i"]
+        h = _Hits(*[F[q].data_ptr() for q in range(8)], *[I[q].data_ptr() for q in range(3)])
+        st = None
+        stats_t = None
+        if stats:
+            stats_t = torch.zeros((len(STAT_NAMES), n), dtype=torch.int32, device="cuda")
+            st = _Stats(*[stats_t[q].data_ptr() for q in range(len(STAT_NAMES))])
+        opt = (options or TraversalOptions())._c()
+        _check(_lib.dmap_intersect_closest(self._h, ctypes.byref(r), n, ctypes.byref(h), int(prim_mask),
+                                           ctypes.byref(opt), ctypes.byref(st) if st is not None else None,
+                                           _stream_ptr(stream)), "dmap_intersect_closest")
+        res = {"t": F[0], "st": F[1:3], "bary": F[3:5], "ng": F[5:8], "prim": I[0], "texel": I[1:3]}
+        if stats:
+            res["stats"] = {k: stats_t[q] for q, k in enumerate(STAT_NAMES)}
+        return res
+
+    def occluded(self, rays, options=None, prim_mask=-1, stream=None):
+        torch = _torch()
+        r, n = self._rays(rays)
+        occ = torch.empty(n, dtype=torch.uint8, device="cuda")
+        opt = (options or TraversalOptions())._c()
+        _check(_lib.dmap_intersect_any(self._h, ctypes.byref(r), n, ctypes.c_void_p(occ.data_ptr()), int(prim_mask),
+                                       ctypes.byref(opt), _stream_ptr(stream)), "dmap_intersect_any")
+        return occ
+
+    def __del__(self):
+        if getattr(self, "_h", None) and self._h.value and _lib is not None:
+            _lib.dmap_object_destroy(self._h)
+            self._h = None

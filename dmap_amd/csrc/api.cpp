@@ -1,0 +1,742 @@
+// C ABI of libdmap_amd.so (include/dmap_amd.h): contexts, resident tiles,
+// objects, and the intersection entry points. Host-side set-up of each
+// triangle runs in double here, once per object; the per-ray work runs in
+// the HIP kernels of pyramid.hip and traverse.hip.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cfloat>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/dmap_amd.h"
+#include "kernels.h"
+#include "layout.h"
+
+using namespace dmap_amd;
+
+struct dmap_ctx
+{
+    int device = 0;
+};
+
+struct dmap_tile
+{
+    dmap_ctx *ctx = nullptr;
+    int n = 0, n_levels = 0, repeat = 1, build = 0;
+    double strength = 1.0, offset = 0.0;
+    float *texels = nullptr, *nodes = nullptr;
+    float2 *mm = nullptr;
+    float4 *tp = nullptr;
+    float2 *rr = nullptr;
+    double *scratch = nullptr;
+    size_t cells = 0;
+    float root_hmin = 0.f, root_hmax = 0.f;
+    int64_t bytes = 0;
+};
+
+struct dmap_object
+{
+    dmap_ctx *ctx = nullptr;
+    dmap_tile *tile = nullptr;
+    int n_tri = 0;
+    std::vector<TriRecord> host;
+    std::vector<float> boxes_host, h_range_host;
+    TriRecord *tris = nullptr;
+    float *boxes = nullptr;
+};
+
+namespace
+{
+
+thread_local std::string g_error;
+
+int fail(int code, const char *fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_error = buf;
+    return code;
+}
+
+#define HIP_TRY(call)                                                                                                 \
+    do {                                                                                                              \
+        hipError_t e_ = (call);                                                                                       \
+        if (e_ != hipSuccess)                                                                                         \
+            return fail(e_ == hipErrorOutOfMemory ? DMAP_ERR_MEMORY : DMAP_ERR_HIP, "%s: %s", #call,                  \
+                        hipGetErrorString(e_));                                                                       \
+    } while (0)
+
+// ------------------------------------------------------------ host math
+
+struct V2
+{
+    double x, y;
+};
+struct V3
+{
+    double x, y, z;
+};
+inline V3 operator+(V3 a, V3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+inline V3 operator-(V3 a, V3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+inline V3 operator*(double s, V3 a) { return {s * a.x, s * a.y, s * a.z}; }
+inline double dot(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+inline V3 cross(V3 a, V3 b) { return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x}; }
+inline double norm(V3 a) { return std::sqrt(dot(a, a)); }
+inline double comp(V3 a, int q) { return q == 0 ? a.x : q == 1 ? a.y : a.z; }
+
+float f_down(double x)
+{
+    float f = (float)x;
+    return (double)f <= x ? f : std::nextafter(f, -INFINITY);
+}
+float f_up(double x)
+{
+    float f = (float)x;
+    return (double)f >= x ? f : std::nextafter(f, INFINITY);
+}
+
+// Affine form over two cell symbols (affine_form.h:40-61).
+struct AF
+{
+    double c = 0, a = 0, b = 0, k = 0;
+    double dev() const { return std::abs(a) + std::abs(b) + k; }
+};
+AF mul(const AF &x, const AF &r)
+{
+    AF o;
+    double u = x.dev(), v = r.dev();
+    o.c = x.c * r.c;
+    o.a = x.c * r.a + r.c * x.a;
+    o.b = x.c * r.b + r.c * x.b;
+    o.k = std::abs(r.c) * x.k + std::abs(x.c) * r.k + u * v;
+    return o;
+}
+AF scale(double s, const AF &x) { return {x.c * s, x.a * s, x.b * s, x.k * std::abs(s)}; }
+AF add(const AF &x, const AF &y) { return {x.c + y.c, x.a + y.a, x.b + y.b, x.k + y.k}; }
+
+// AffineVec3::normalize (affine_form.cpp:64-95).
+void normalize(AF N[3])
+{
+    double dev[3], cc = 0, cu = 0, cv = 0, ck = 0, dd = 0;
+    for (int q = 0; q < 3; ++q) {
+        dev[q] = N[q].dev();
+        dd += dev[q] * dev[q];
+    }
+    for (int q = 0; q < 3; ++q) {
+        cc += N[q].c * N[q].c;
+        cu += N[q].c * N[q].a;
+        cv += N[q].c * N[q].b;
+        ck += N[q].c * N[q].k;
+    }
+    double offset = 0.5 * dd;
+    AF l{cc + offset, 2.0 * cu, 2.0 * cv, 2.0 * std::abs(ck) + offset};
+    double lo = l.c - l.dev(), hi = l.c + l.dev();
+    if (lo <= 0.0) {
+        for (int q = 0; q < 3; ++q)
+            N[q] = AF{0, 0, 0, 1};
+        return;
+    }
+    auto g = [](double x) { return 1.0 / std::sqrt(x); };
+    double alpha = -0.5 * g(hi) * g(hi) * g(hi);
+    double beta = 0.5 * (g(lo) + g(hi) - alpha * (lo + hi));
+    double delta = 0.5 * std::abs(g(lo) - g(hi) - alpha * (lo - hi));
+    AF rs{alpha * l.c + beta, alpha * l.a, alpha * l.b, std::abs(alpha) * l.k + delta};
+    for (int q = 0; q < 3; ++q)
+        N[q] = mul(rs, N[q]);
+}
+
+// The host-side description of one triangle, in double.
+struct HostTri
+{
+    V3 p0, p1, p2, q0, e1, e2, m0, Mu, Mv;
+    V2 t0, t1, t2;
+    double J[4];
+};
+
+bool make_tri(const double *pos, const double *nrm, const double *uv, HostTri &t)
+{
+    // BaseTriangle, general chart (displaced_surface.cpp:54-91).
+    t.p0 = {pos[0], pos[1], pos[2]};
+    t.p1 = {pos[3], pos[4], pos[5]};
+    t.p2 = {pos[6], pos[7], pos[8]};
+    V3 n0 = {nrm[0], nrm[1], nrm[2]}, n1 = {nrm[3], nrm[4], nrm[5]}, n2 = {nrm[6], nrm[7], nrm[8]};
+    t.t0 = {uv[0], uv[1]};
+    t.t1 = {uv[2], uv[3]};
+    t.t2 = {uv[4], uv[5]};
+    double a = t.t1.x - t.t0.x, b = t.t2.x - t.t0.x, c = t.t1.y - t.t0.y, d = t.t2.y - t.t0.y;
+    double det = a * d - b * c;
+    if (!(std::abs(det) > 1e-30))
+        return false;
+    t.J[0] = d / det;
+    t.J[1] = -b / det;
+    t.J[2] = -c / det;
+    t.J[3] = a / det;
+    V3 eb1 = t.p1 - t.p0, eb2 = t.p2 - t.p0, Mb1 = n1 - n0, Mb2 = n2 - n0;
+    t.e1 = t.J[0] * eb1 + t.J[2] * eb2;
+    t.e2 = t.J[1] * eb1 + t.J[3] * eb2;
+    t.Mu = t.J[0] * Mb1 + t.J[2] * Mb2;
+    t.Mv = t.J[1] * Mb1 + t.J[3] * Mb2;
+    double c0 = -(t.J[0] * t.t0.x + t.J[1] * t.t0.y), c1 = -(t.J[2] * t.t0.x + t.J[3] * t.t0.y);
+    t.q0 = t.p0 + (c0 * eb1 + c1 * eb2);
+    t.m0 = n0 + (c0 * Mb1 + c1 * Mb2);
+    return true;
+}
+
+bool fill_record(const HostTri &h, int n_leaf, TriRecord &R)
+{
+    std::memset(&R, 0, sizeof R);
+    const V3 *vs[6] = {&h.q0, &h.e1, &h.e2, &h.m0, &h.Mu, &h.Mv};
+    double *ds[6] = {R.q0, R.e1, R.e2, R.m0, R.Mu, R.Mv};
+    for (int v = 0; v < 6; ++v)
+        for (int q = 0; q < 3; ++q)
+            ds[v][q] = comp(*vs[v], q);
+    // TangentFrame (node_bounds.cpp:13-23): to_tangent = [e1 e2 ng]^-1.
+    V3 ng = cross(h.e1, h.e2);
+    double ngl = norm(ng);
+    if (!(ngl > 0.0))
+        return false;
+    ng = (1.0 / ngl) * ng;
+    double m[3][3];
+    for (int r = 0; r < 3; ++r) {
+        m[r][0] = comp(h.e1, r);
+        m[r][1] = comp(h.e2, r);
+        m[r][2] = comp(ng, r);
+    }
+    double c00 = m[1][1] * m[2][2] - m[1][2] * m[2][1], c01 = m[1][2] * m[2][0] - m[1][0] * m[2][2],
+           c02 = m[1][0] * m[2][1] - m[1][1] * m[2][0];
+    double det = m[0][0] * c00 + m[0][1] * c01 + m[0][2] * c02;
+    if (!(std::abs(det) > 0.0))
+        return false;
+    double inv = 1.0 / det;
+    double T[9] = {c00 * inv,
+                   (m[0][2] * m[2][1] - m[0][1] * m[2][2]) * inv,
+                   (m[0][1] * m[1][2] - m[0][2] * m[1][1]) * inv,
+                   c01 * inv,
+                   (m[0][0] * m[2][2] - m[0][2] * m[2][0]) * inv,
+                   (m[0][2] * m[1][0] - m[0][0] * m[1][2]) * inv,
+                   c02 * inv,
+                   (m[0][1] * m[2][0] - m[0][0] * m[2][1]) * inv,
+                   (m[0][0] * m[1][1] - m[0][1] * m[1][0]) * inv};
+    for (int q = 0; q < 9; ++q) {
+        R.T[q] = T[q];
+        R.f_T[q] = (float)T[q];
+    }
+    for (int q = 0; q < 4; ++q)
+        R.J[q] = h.J[q];
+    R.t0[0] = h.t0.x;
+    R.t0[1] = h.t0.y;
+    // UvTriangle (uv_clip.cpp:10-25).
+    V2 t[3] = {h.t0, h.t1, h.t2};
+    double signed_area = 0.5 * ((h.t1.x - h.t0.x) * (h.t2.y - h.t0.y) - (h.t1.y - h.t0.y) * (h.t2.x - h.t0.x));
+    double orient = signed_area >= 0.0 ? 1.0 : -1.0;
+    for (int k = 0; k < 3; ++k) {
+        V2 e = {t[(k + 1) % 3].x - t[k].x, t[(k + 1) % 3].y - t[k].y};
+        double nx = orient * -e.y, ny = orient * e.x;
+        R.dn_d[2 * k] = nx;
+        R.dn_d[2 * k + 1] = ny;
+        R.dd_d[k] = nx * t[k].x + ny * t[k].y;
+        R.dn[2 * k] = (float)nx;
+        R.dn[2 * k + 1] = (float)ny;
+        R.dd[k] = (float)R.dd_d[k];
+    }
+    R.dlo_d[0] = std::min(std::min(h.t0.x, h.t1.x), h.t2.x);
+    R.dlo_d[1] = std::min(std::min(h.t0.y, h.t1.y), h.t2.y);
+    R.dhi_d[0] = std::max(std::max(h.t0.x, h.t1.x), h.t2.x);
+    R.dhi_d[1] = std::max(std::max(h.t0.y, h.t1.y), h.t2.y);
+    for (int q = 0; q < 2; ++q) {
+        R.dlo[q] = (float)R.dlo_d[q];
+        R.dhi[q] = (float)R.dhi_d[q];
+    }
+    // footprint_roots (descent_sampler.cpp:66-83) and the root cells of
+    // the DisplacedIntersector ctor (displaced_intersector.cpp:139-149).
+    int level = 0;
+    int64_t i0 = 0, j0 = 0;
+    while (true) {
+        double w = std::ldexp(1.0, level) / n_leaf;
+        int64_t ilo = (int64_t)std::floor(R.dlo_d[0] / w), ihi = (int64_t)std::ceil(R.dhi_d[0] / w) - 1;
+        int64_t jlo = (int64_t)std::floor(R.dlo_d[1] / w), jhi = (int64_t)std::ceil(R.dhi_d[1] / w) - 1;
+        if (ihi - ilo <= 1 && jhi - jlo <= 1) {
+            i0 = ilo;
+            j0 = jlo;
+            break;
+        }
+        if (++level > 40)
+            return false;
+    }
+    double w = std::ldexp(1.0, level) / n_leaf;
+    int64_t i_lo = (int64_t)std::floor(R.dlo_d[0] / w), i_hi = (int64_t)std::ceil(R.dhi_d[0] / w) - 1;
+    int64_t j_lo = (int64_t)std::floor(R.dlo_d[1] / w), j_hi = (int64_t)std::ceil(R.dhi_d[1] / w) - 1;
+    if (!(i_lo == i0 && j_lo == j0 && i_hi <= i0 + 1 && j_hi <= j0 + 1))
+        return false;
+    if (std::abs(i_lo) > (1 << 28) || std::abs(j_lo) > (1 << 28))
+        return false;
+    R.root_level = level;
+    R.n_roots = 0;
+    for (int64_t j = j_lo; j <= j_hi; ++j)
+        for (int64_t i = i_lo; i <= i_hi; ++i) {
+            R.root_i[R.n_roots] = (int)i;
+            R.root_j[R.n_roots] = (int)j;
+            ++R.n_roots;
+        }
+    // A constant interpolated normal gives one normalized form for every
+    // cell (node_bounds.cpp:100-103 with Mu = Mv = 0).
+    R.const_normal = (h.Mu.x == 0.0 && h.Mu.y == 0.0 && h.Mu.z == 0.0 && h.Mv.x == 0.0 && h.Mv.y == 0.0 &&
+                      h.Mv.z == 0.0)
+                         ? 1
+                         : 0;
+    {
+        AF N[3] = {AF{h.m0.x, 0, 0, 0}, AF{h.m0.y, 0, 0, 0}, AF{h.m0.z, 0, 0, 0}};
+        normalize(N);
+        double kmax = 0;
+        for (int q = 0; q < 3; ++q) {
+            R.nhat_d[q] = N[q].c;
+            R.nhat[q] = (float)N[q].c;
+            kmax = std::max(kmax, N[q].k);
+        }
+        R.nhat_kd = kmax;
+        R.nhat_k = f_up(kmax);
+        for (int r = 0; r < 3; ++r) {
+            double c = 0, k = 0;
+            for (int q = 0; q < 3; ++q) {
+                c += T[3 * r + q] * N[q].c;
+                k += std::abs(T[3 * r + q]) * N[q].k;
+            }
+            R.mc_d[r] = c;
+            R.mk_d[r] = k;
+            R.mc[r] = (float)c;
+            R.mk[r] = f_up((k + std::abs(c - (double)R.mc[r])) * (1.0 + 0x1p-50));
+        }
+    }
+    for (int q = 0; q < 3; ++q) {
+        R.f_m0[q] = (float)comp(h.m0, q);
+        R.f_Mu[q] = (float)comp(h.Mu, q);
+        R.f_Mv[q] = (float)comp(h.Mv, q);
+    }
+    // residual_tolerance = 1e-10 x mean edge (displaced_intersector.cpp:150,
+    // displaced_surface.cpp:168-171).
+    double me = (norm(h.p1 - h.p0) + norm(h.p2 - h.p0) + norm(h.p2 - h.p1)) / 3.0;
+    R.residual_tolerance = 1e-10 * me;
+    return true;
+}
+
+// triangle_aabb (node_bounds.cpp:251-269), then widened for fp32 rays as
+// rtc_bounds does (displaced_intersector.cpp:454-464).
+void triangle_box(const HostTri &tri, double h_min, double h_max, float out[6])
+{
+    const V2 t[3] = {tri.t0, tri.t1, tri.t2};
+    AF h{0.5 * (h_min + h_max), 0, 0, 0.5 * (h_max - h_min)};
+    double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (int k = 0; k < 3; ++k) {
+        V2 c = {0.5 * t[k].x + 0.25 * t[(k + 1) % 3].x + 0.25 * t[(k + 2) % 3].x,
+                0.5 * t[k].y + 0.25 * t[(k + 1) % 3].y + 0.25 * t[(k + 2) % 3].y};
+        V2 du = {0.25 * (t[(k + 1) % 3].x - t[k].x), 0.25 * (t[(k + 1) % 3].y - t[k].y)};
+        V2 dv = {0.25 * (t[(k + 2) % 3].x - t[k].x), 0.25 * (t[(k + 2) % 3].y - t[k].y)};
+        V3 P = tri.q0 + c.x * tri.e1 + c.y * tri.e2;
+        V3 Pa = du.x * tri.e1 + du.y * tri.e2, Pb = dv.x * tri.e1 + dv.y * tri.e2;
+        V3 M = tri.m0 + c.x * tri.Mu + c.y * tri.Mv;
+        V3 Ma = du.x * tri.Mu + du.y * tri.Mv, Mb = dv.x * tri.Mu + dv.y * tri.Mv;
+        AF N[3], Pf[3];
+        for (int q = 0; q < 3; ++q) {
+            N[q] = AF{comp(M, q), comp(Ma, q), comp(Mb, q), 0};
+            Pf[q] = AF{comp(P, q), comp(Pa, q), comp(Pb, q), 0};
+        }
+        normalize(N);
+        for (int q = 0; q < 3; ++q) {
+            AF S = add(Pf[q], mul(h, N[q]));
+            lo[q] = std::min(lo[q], S.c - S.dev());
+            hi[q] = std::max(hi[q], S.c + S.dev());
+        }
+    }
+    double mag = 0;
+    for (int q = 0; q < 3; ++q)
+        mag = std::max(mag, std::max(std::abs(lo[q]), std::abs(hi[q])));
+    double pad = 8.0 * (double)FLT_EPSILON * (1.0 + mag);
+    for (int q = 0; q < 3; ++q) {
+        out[q] = f_down(lo[q] - pad);
+        out[3 + q] = f_up(hi[q] + pad);
+    }
+}
+
+bool is_pow2(int n) { return n >= 1 && (n & (n - 1)) == 0; }
+
+} // namespace
+
+extern "C" {
+
+const char *dmap_last_error(void) { return g_error.c_str(); }
+const char *dmap_version(void) { return "dmap_amd 0.1 gfx950"; }
+
+int dmap_ctx_create(int device, dmap_ctx **out)
+{
+    if (!out)
+        return fail(DMAP_ERR_INVALID, "dmap_ctx_create: out is NULL");
+    int count = 0;
+    HIP_TRY(hipGetDeviceCount(&count));
+    if (device < 0 || device >= count)
+        return fail(DMAP_ERR_INVALID, "dmap_ctx_create: device %d of %d", device, count);
+    HIP_TRY(hipSetDevice(device));
+    dmap_ctx *c = new dmap_ctx;
+    c->device = device;
+    *out = c;
+    return DMAP_OK;
+}
+
+void dmap_ctx_destroy(dmap_ctx *ctx) { delete ctx; }
+
+static void tile_free(dmap_tile *t)
+{
+    if (!t)
+        return;
+    hipFree(t->texels);
+    hipFree(t->nodes);
+    hipFree(t->mm);
+    hipFree(t->tp);
+    hipFree(t->rr);
+    hipFree(t->scratch);
+    delete t;
+}
+
+static int tile_build(dmap_tile *t, hipStream_t stream)
+{
+    HIP_TRY(hipSetDevice(t->ctx->device));
+    PyramidOut P{t->mm, t->tp, t->rr, t->n, t->n_levels, t->strength, t->offset};
+    hipError_t e = t->build == 0 ? launch_build_fold(P, t->nodes, t->scratch, stream)
+                                 : launch_build_direct(P, t->nodes, stream);
+    HIP_TRY(e);
+    return DMAP_OK;
+}
+
+int dmap_tile_create(dmap_ctx *ctx, const float *texels, int n, int texels_on_device, int repeat, double strength,
+                     double offset, int build, void *stream_, dmap_tile **out)
+{
+    if (!ctx || !texels || !out)
+        return fail(DMAP_ERR_INVALID, "dmap_tile_create: NULL argument");
+    if (!is_pow2(n) || n > 16384)
+        return fail(DMAP_ERR_INVALID, "dmap_tile_create: the tile needs 2^L texels per side (<= 16384), got %d", n);
+    if (build != 0 && build != 1)
+        return fail(DMAP_ERR_INVALID, "dmap_tile_create: unknown pyramid build %d, expected 0 fold or 1 direct", build);
+    if (!std::isfinite(strength) || !std::isfinite(offset))
+        return fail(DMAP_ERR_INVALID, "dmap_tile_create: non-finite strength or offset");
+    hipStream_t stream = (hipStream_t)stream_;
+    HIP_TRY(hipSetDevice(ctx->device));
+    dmap_tile *t = new dmap_tile;
+    t->ctx = ctx;
+    t->n = n;
+    t->n_levels = 1;
+    while ((1 << t->n_levels) <= n)
+        ++t->n_levels;
+    t->repeat = repeat ? 1 : 0;
+    t->build = build;
+    t->strength = strength;
+    t->offset = offset;
+    t->cells = ((size_t)4 * n * n - 1) / 3;
+    size_t nn = (size_t)n * n, nodes = (size_t)(n + 1) * (n + 1);
+    size_t scratch = fold_scratch_doubles(n);
+    hipError_t e;
+    if ((e = hipMalloc(&t->texels, nn * 4)) != hipSuccess || (e = hipMalloc(&t->nodes, nodes * 4)) != hipSuccess ||
+        (e = hipMalloc(&t->mm, t->cells * 8)) != hipSuccess || (e = hipMalloc(&t->tp, t->cells * 16)) != hipSuccess ||
+        (e = hipMalloc(&t->rr, t->cells * 8)) != hipSuccess ||
+        (e = hipMalloc(&t->scratch, scratch * 8)) != hipSuccess) {
+        tile_free(t);
+        return fail(DMAP_ERR_MEMORY, "dmap_tile_create: device allocation failed (%s)", hipGetErrorString(e));
+    }
+    t->bytes = (int64_t)(nn * 4 + nodes * 4 + t->cells * 32 + scratch * 8);
+    e = hipMemcpyAsync(t->texels, texels, nn * 4, texels_on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice,
+                       stream);
+    if (e == hipSuccess)
+        e = launch_close_tile(t->texels, t->nodes, n, t->repeat, stream);
+    if (e != hipSuccess) {
+        tile_free(t);
+        return fail(DMAP_ERR_HIP, "dmap_tile_create: %s", hipGetErrorString(e));
+    }
+    int rc = tile_build(t, stream);
+    if (rc != DMAP_OK) {
+        tile_free(t);
+        return rc;
+    }
+    float2 root;
+    e = hipMemcpyAsync(&root, t->mm, sizeof root, hipMemcpyDeviceToHost, stream);
+    if (e == hipSuccess)
+        e = hipStreamSynchronize(stream);
+    if (e != hipSuccess) {
+        tile_free(t);
+        return fail(DMAP_ERR_HIP, "dmap_tile_create: %s", hipGetErrorString(e));
+    }
+    t->root_hmin = root.x;
+    t->root_hmax = root.y;
+    *out = t;
+    return DMAP_OK;
+}
+
+int dmap_tile_rebuild(dmap_tile *tile, void *stream)
+{
+    if (!tile)
+        return fail(DMAP_ERR_INVALID, "dmap_tile_rebuild: NULL tile");
+    return tile_build(tile, (hipStream_t)stream);
+}
+
+int dmap_tile_info(const dmap_tile *tile, int *n_leaf, int *n_levels, int64_t *device_bytes)
+{
+    if (!tile)
+        return fail(DMAP_ERR_INVALID, "dmap_tile_info: NULL tile");
+    if (n_leaf)
+        *n_leaf = tile->n;
+    if (n_levels)
+        *n_levels = tile->n_levels;
+    if (device_bytes)
+        *device_bytes = tile->bytes;
+    return DMAP_OK;
+}
+
+int dmap_tile_read_level(const dmap_tile *tile, int level, float *out)
+{
+    if (!tile || !out)
+        return fail(DMAP_ERR_INVALID, "dmap_tile_read_level: NULL argument");
+    if (level < 0 || level >= tile->n_levels)
+        return fail(DMAP_ERR_INVALID, "dmap_tile_read_level: level %d of %d", level, tile->n_levels);
+    HIP_TRY(hipSetDevice(tile->ctx->device));
+    uint32_t m = (uint32_t)tile->n >> level;
+    size_t cnt = (size_t)m * m, base = level_base(m);
+    std::vector<float2> mm(cnt), rr(cnt);
+    std::vector<float4> tp(cnt);
+    HIP_TRY(hipMemcpy(mm.data(), tile->mm + base, cnt * 8, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(tp.data(), tile->tp + base, cnt * 16, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(rr.data(), tile->rr + base, cnt * 8, hipMemcpyDeviceToHost));
+    for (uint32_t j = 0; j < m; ++j)
+        for (uint32_t i = 0; i < m; ++i) {
+            size_t src = morton2(i, j), dst = (size_t)j * m + i;
+            out[0 * cnt + dst] = tp[src].x;
+            out[1 * cnt + dst] = tp[src].y;
+            out[2 * cnt + dst] = tp[src].z;
+            out[3 * cnt + dst] = tp[src].w;
+            out[4 * cnt + dst] = rr[src].x;
+            out[5 * cnt + dst] = rr[src].y;
+            out[6 * cnt + dst] = mm[src].x;
+            out[7 * cnt + dst] = mm[src].y;
+        }
+    return DMAP_OK;
+}
+
+int dmap_tile_read_nodes(const dmap_tile *tile, float *out)
+{
+    if (!tile || !out)
+        return fail(DMAP_ERR_INVALID, "dmap_tile_read_nodes: NULL argument");
+    HIP_TRY(hipSetDevice(tile->ctx->device));
+    size_t nodes = (size_t)(tile->n + 1) * (tile->n + 1);
+    HIP_TRY(hipMemcpy(out, tile->nodes, nodes * 4, hipMemcpyDeviceToHost));
+    return DMAP_OK;
+}
+
+void dmap_tile_destroy(dmap_tile *tile)
+{
+    if (tile)
+        hipSetDevice(tile->ctx->device);
+    tile_free(tile);
+}
+
+static PyramidView view_of(const dmap_tile *t)
+{
+    PyramidView v;
+    v.mm = t->mm;
+    v.tp = t->tp;
+    v.rr = t->rr;
+    v.nodes = t->nodes;
+    v.n_leaf = t->n;
+    v.n_levels = t->n_levels;
+    v.repeat = t->repeat;
+    v.scale = (float)t->strength;
+    v.offset = (float)t->offset;
+    v.scale_d = t->strength;
+    v.offset_d = t->offset;
+    v.root_hmin = t->root_hmin;
+    v.root_hmax = t->root_hmax;
+    return v;
+}
+
+static void object_free(dmap_object *o)
+{
+    if (!o)
+        return;
+    hipFree(o->tris);
+    hipFree(o->boxes);
+    delete o;
+}
+
+int dmap_object_create(dmap_ctx *ctx, dmap_tile *tile, int n_tri, const double *positions, const double *normals,
+                       const double *uvs, dmap_object **out)
+{
+    if (!ctx || !tile || !positions || !normals || !uvs || !out || n_tri <= 0)
+        return fail(DMAP_ERR_INVALID, "dmap_object_create: bad argument");
+    if (tile->ctx->device != ctx->device)
+        return fail(DMAP_ERR_INVALID, "dmap_object_create: tile lives on another device");
+    HIP_TRY(hipSetDevice(ctx->device));
+    std::vector<HostTri> tris(n_tri);
+    dmap_object *o = new dmap_object;
+    o->ctx = ctx;
+    o->tile = tile;
+    o->n_tri = n_tri;
+    o->host.resize(n_tri);
+    for (int k = 0; k < n_tri; ++k) {
+        if (!make_tri(positions + 9 * (size_t)k, normals + 9 * (size_t)k, uvs + 6 * (size_t)k, tris[k])) {
+            delete o;
+            return fail(DMAP_ERR_INVALID, "dmap_object_create: triangle %d is degenerate in the texture plane", k);
+        }
+        if (!fill_record(tris[k], tile->n, o->host[k])) {
+            delete o;
+            return fail(DMAP_ERR_INVALID, "dmap_object_create: triangle %d has a degenerate frame or footprint", k);
+        }
+    }
+    hipError_t e;
+    if ((e = hipMalloc(&o->tris, sizeof(TriRecord) * n_tri)) != hipSuccess ||
+        (e = hipMalloc(&o->boxes, sizeof(float) * 6 * n_tri)) != hipSuccess) {
+        object_free(o);
+        return fail(DMAP_ERR_MEMORY, "dmap_object_create: %s", hipGetErrorString(e));
+    }
+    float *h_range = nullptr;
+    e = hipMalloc(&h_range, sizeof(float) * 2 * n_tri);
+    if (e == hipSuccess)
+        e = hipMemcpy(o->tris, o->host.data(), sizeof(TriRecord) * n_tri, hipMemcpyHostToDevice);
+    if (e == hipSuccess)
+        e = launch_height_range(view_of(tile), o->tris, n_tri, h_range, nullptr);
+    o->h_range_host.resize(2 * (size_t)n_tri);
+    if (e == hipSuccess)
+        e = hipMemcpy(o->h_range_host.data(), h_range, sizeof(float) * 2 * n_tri, hipMemcpyDeviceToHost);
+    hipFree(h_range);
+    if (e != hipSuccess) {
+        object_free(o);
+        return fail(DMAP_ERR_HIP, "dmap_object_create: %s", hipGetErrorString(e));
+    }
+    o->boxes_host.resize(6 * (size_t)n_tri);
+    for (int k = 0; k < n_tri; ++k) {
+        float lo = o->h_range_host[2 * k], hi = o->h_range_host[2 * k + 1];
+        if (!(lo <= hi)) {
+            object_free(o);
+            return fail(DMAP_ERR_INVALID, "dmap_object_create: triangle %d's footprint holds no cell", k);
+        }
+        triangle_box(tris[k], lo, hi, &o->boxes_host[6 * k]);
+        for (int q = 0; q < 3; ++q) {
+            o->host[k].aabb_lo[q] = o->boxes_host[6 * k + q];
+            o->host[k].aabb_hi[q] = o->boxes_host[6 * k + 3 + q];
+        }
+    }
+    e = hipMemcpy(o->boxes, o->boxes_host.data(), sizeof(float) * 6 * n_tri, hipMemcpyHostToDevice);
+    if (e == hipSuccess)
+        e = hipMemcpy(o->tris, o->host.data(), sizeof(TriRecord) * n_tri, hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        object_free(o);
+        return fail(DMAP_ERR_HIP, "dmap_object_create: %s", hipGetErrorString(e));
+    }
+    *out = o;
+    return DMAP_OK;
+}
+
+int dmap_object_triangle_info(const dmap_object *obj, int k, int *root_level, int *n_roots, int32_t *roots,
+                              float *aabb, float *h_range)
+{
+    if (!obj || k < 0 || k >= obj->n_tri)
+        return fail(DMAP_ERR_INVALID, "dmap_object_triangle_info: bad triangle %d", k);
+    const TriRecord &R = obj->host[k];
+    if (root_level)
+        *root_level = R.root_level;
+    if (n_roots)
+        *n_roots = R.n_roots;
+    if (roots)
+        for (int r = 0; r < 4; ++r) {
+            roots[2 * r] = r < R.n_roots ? R.root_i[r] : 0;
+            roots[2 * r + 1] = r < R.n_roots ? R.root_j[r] : 0;
+        }
+    if (aabb)
+        std::memcpy(aabb, &obj->boxes_host[6 * k], 6 * sizeof(float));
+    if (h_range) {
+        h_range[0] = obj->h_range_host[2 * k];
+        h_range[1] = obj->h_range_host[2 * k + 1];
+    }
+    return DMAP_OK;
+}
+
+void dmap_object_destroy(dmap_object *obj)
+{
+    if (obj)
+        hipSetDevice(obj->ctx->device);
+    object_free(obj);
+}
+
+static int check_options(const dmap_options *opt, dmap_options &o)
+{
+    o = dmap_options{2, 1, 0, 0};
+    if (opt)
+        o = *opt;
+    if (o.bound < 0 || o.bound > 2 || o.leaf < 0 || o.leaf > 1 || o.precision < 0 || o.precision > 1)
+        return fail(DMAP_ERR_INVALID, "bad traversal options (bound %d, leaf %d, precision %d)", o.bound, o.leaf,
+                    o.precision);
+    return DMAP_OK;
+}
+
+static int trace(const dmap_object *obj, const dmap_rays *rays, int64_t n, const dmap_hits *hits, uint8_t *occluded,
+                 int prim_mask, const dmap_options *options, const dmap_ray_stats *stats, void *stream)
+{
+    if (!obj || !rays)
+        return fail(DMAP_ERR_INVALID, "intersect: NULL object or rays");
+    if (n < 0)
+        return fail(DMAP_ERR_INVALID, "intersect: negative ray count");
+    if (n == 0)
+        return DMAP_OK;
+    if (!rays->ox || !rays->oy || !rays->oz || !rays->dx || !rays->dy || !rays->dz || !rays->tmin || !rays->tmax)
+        return fail(DMAP_ERR_INVALID, "intersect: a ray array is NULL");
+    if (prim_mask < -1 || prim_mask >= obj->n_tri)
+        return fail(DMAP_ERR_INVALID, "intersect: prim_mask %d of %d triangles", prim_mask, obj->n_tri);
+    dmap_options o;
+    int rc = check_options(options, o);
+    if (rc != DMAP_OK)
+        return rc;
+    HIP_TRY(hipSetDevice(obj->ctx->device));
+    TraceParams p;
+    std::memset(&p, 0, sizeof p);
+    p.pyr = view_of(obj->tile);
+    p.tris = obj->tris;
+    p.boxes = obj->boxes;
+    p.n_tri = obj->n_tri;
+    p.prim_mask = prim_mask;
+    p.bound = o.bound;
+    p.slab_max_level = o.slab_max_level;
+    p.leaf = o.leaf;
+    p.any_hit = occluded ? 1 : 0;
+    p.n_rays = n;
+    p.rays = RaysDev{rays->ox, rays->oy, rays->oz, rays->dx, rays->dy, rays->dz, rays->tmin, rays->tmax};
+    if (hits)
+        p.hits = HitsDev{hits->t,  hits->s,  hits->tt,   hits->bu,      hits->bv,     hits->nx,
+                         hits->ny, hits->nz, hits->prim, hits->texel_i, hits->texel_j};
+    p.occluded = occluded;
+    if (stats)
+        p.stats = StatsDev{stats->triangle_tests, stats->node_tests, stats->leaf_tests, stats->sub_squares,
+                           stats->newton_runs, stats->newton_iters};
+    HIP_TRY(launch_trace(p, o.precision, (hipStream_t)stream));
+    return DMAP_OK;
+}
+
+int dmap_intersect_closest(const dmap_object *obj, const dmap_rays *rays, int64_t n, const dmap_hits *hits,
+                           int prim_mask, const dmap_options *options, const dmap_ray_stats *stats, void *stream)
+{
+    if (!hits || !hits->t)
+        return fail(DMAP_ERR_INVALID, "dmap_intersect_closest: hits->t is NULL");
+    return trace(obj, rays, n, hits, nullptr, prim_mask, options, stats, stream);
+}
+
+int dmap_intersect_any(const dmap_object *obj, const dmap_rays *rays, int64_t n, uint8_t *occluded, int prim_mask,
+                       const dmap_options *options, void *stream)
+{
+    if (!occluded)
+        return fail(DMAP_ERR_INVALID, "dmap_intersect_any: occluded is NULL");
+    return trace(obj, rays, n, nullptr, occluded, prim_mask, options, nullptr, stream);
+}
+
+} // extern "C"

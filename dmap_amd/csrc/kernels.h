@@ -1,0 +1,68 @@
+// Launch wrappers of the HIP kernels, called by the C-ABI layer (api.cpp).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#include "layout.h"
+
+namespace dmap_amd
+{
+
+// Writable view of a pyramid during its build.
+struct PyramidOut
+{
+    float2 *mm;
+    float4 *tp;
+    float2 *rr;
+    int n_leaf, n_levels;
+    double scale, offset;
+};
+
+hipError_t launch_close_tile(const float *texels, float *nodes, int n, int repeat, hipStream_t stream);
+size_t fold_scratch_doubles(int n);
+hipError_t launch_build_fold(const PyramidOut &P, const float *nodes, double *scratch, hipStream_t stream);
+hipError_t launch_build_direct(const PyramidOut &P, const float *nodes, hipStream_t stream);
+
+// triangle_height_range (node_bounds.cpp:218-249) for every triangle: one
+// thread per triangle walks the min-max channel over its footprint.
+hipError_t launch_height_range(const PyramidView &P, const TriRecord *tris, int n_tri, float *h_range,
+                               hipStream_t stream);
+
+// Ray and hit buffers (device pointers, SoA).
+struct RaysDev
+{
+    const float *ox, *oy, *oz, *dx, *dy, *dz, *tmin, *tmax;
+};
+struct HitsDev
+{
+    float *t, *s, *tt, *bu, *bv, *nx, *ny, *nz;
+    int32_t *prim, *texel_i, *texel_j;
+};
+struct StatsDev
+{
+    int32_t *triangle_tests, *node_tests, *leaf_tests, *sub_squares, *newton_runs, *newton_iters;
+};
+
+struct TraceParams
+{
+    PyramidView pyr;
+    const TriRecord *tris;
+    const float *boxes; // n_tri x 6: lo xyz, hi xyz
+    int n_tri;
+    int prim_mask; // -1: every triangle; k: triangle k only
+    int bound;     // 0 box, 1 slab, 2 both
+    int slab_max_level;
+    int leaf; // 0 newton, 1 two-triangle
+    int any_hit;
+    int64_t n_rays;
+    RaysDev rays;
+    HitsDev hits;
+    uint8_t *occluded;
+    StatsDev stats;
+};
+
+// Closest hit / any hit over the object; precision 0 = fp32, 1 = fp64.
+hipError_t launch_trace(const TraceParams &p, int precision, hipStream_t stream);
+
+} // namespace dmap_amd

@@ -1,0 +1,1253 @@
+// Closest-hit and any-hit traversal of displaced triangles on gfx950
+// (kernels #2 and #3): the stackless quadtree walk with the box / slab
+// node tests and the certified Newton leaf, one ray per lane.
+//
+// Follows code/cpp/src/displaced_intersector.cpp (walk :85-121, :175-237;
+// Newton leaf :239-388; two-triangle leaf :390-426; flat object loop
+// :578-599) and node_bounds.cpp (:25-44, :83-211), restated for a
+// wavefront:
+//   * the per-lane state machine (next triangle / walk / leaf) runs the
+//     walk until every lane of the wave has a leaf pending or is done, then
+//     the leaves run together (while-while), so the expensive leaf search
+//     is not serialised lane by lane;
+//   * the leaf's sub-square search keeps its depth-first stack (at most 16
+//     entries) in LDS, lane-interleaved;
+//   * node data come from the Morton-ordered fp32 pyramid: a box test reads
+//     one float2, a slab test one float4 more.
+//
+// Precision. The kernel is templated on the working type R. R = float is
+// the product path; R = double is the FP64 device run of the same solve.
+// Every interval the float path computes is widened by its rounding error
+// (a few ulps of the magnitudes involved), so the float tests keep every
+// node and sub-square the double tests keep. Quantities that cancel are
+// formed in double once per walk or per leaf: the ray in tangent space,
+// and the leaf's anchor projections d1 . (P0 - o), d2 . (P0 - o),
+// dn . (P0 - o) about the texel corner P0; the Newton iteration then runs
+// in texel-local coordinates (a, b) in [0, 1]^2, where its terms are small.
+#include <hip/hip_runtime.h>
+
+#include "kernels.h"
+#include "layout.h"
+
+namespace dmap_amd
+{
+
+template <class R> struct Num;
+template <> struct Num<float>
+{
+    static constexpr float eps = 0x1p-24f;
+    static constexpr float tol_rel = 16.0f * 0x1p-24f; // interval_tolerance's relative room
+};
+template <> struct Num<double>
+{
+    static constexpr double eps = 0x1p-53;
+    static constexpr double tol_rel = 1e-9; // node_bounds.cpp:128
+};
+
+constexpr int KPAD = 8;        // ulps of padding per rounded interval
+constexpr int KNEWTON = 16;    // ulps of the residual's magnitude accepted as converged
+constexpr int SUBDIV_DEPTH = 5; // displaced_intersector.h:132
+constexpr int MAX_ITERS = 20;   // displaced_intersector.h:129
+constexpr int STACK = 16;       // 4 + 3 x 4 entries for a depth-5 search
+constexpr int BLOCK = 256;
+
+template <class R> __device__ inline R pad_of(R magnitude) { return (R)KPAD * Num<R>::eps * magnitude; }
+template <class R> __device__ inline R tolerance(R t_scale) { return Num<R>::tol_rel * fmax((R)1, t_scale); }
+
+// Aabb3::clip (node_bounds.cpp:25-44) on one axis, widened by rounding room.
+template <class R> __device__ inline bool clip_axis(R o, R d, R inv, R lo, R hi, R &t0, R &t1)
+{
+    R pad = pad_of<R>(fabs(o) + fmax(fabs(lo), fabs(hi)));
+    lo -= pad;
+    hi += pad;
+    if (d != (R)0) {
+        R ta = (lo - o) * inv, tb = (hi - o) * inv;
+        if (ta > tb) {
+            R x = ta;
+            ta = tb;
+            tb = x;
+        }
+        ta -= pad_of<R>(fabs(ta));
+        tb += pad_of<R>(fabs(tb));
+        t0 = fmax(t0, ta);
+        t1 = fmin(t1, tb);
+    } else if (o < lo || o > hi) {
+        return false;
+    }
+    return true;
+}
+
+template <class R> __device__ inline bool close_interval(R &t0, R &t1)
+{
+    if (t0 > t1 + tolerance<R>(fmax(fabs(t0), fabs(t1))))
+        return false;
+    if (t0 > t1) {
+        R x = t0;
+        t0 = t1;
+        t1 = x;
+    }
+    return true;
+}
+
+struct Iv
+{
+    double lo, hi;
+};
+template <class R> struct IvR
+{
+    R lo, hi;
+    __device__ R mag() const { return fmax(fabs(lo), fabs(hi)); }
+};
+template <class R> __device__ inline IvR<R> smul(R s, IvR<R> a)
+{
+    return s >= (R)0 ? IvR<R>{s * a.lo, s * a.hi} : IvR<R>{s * a.hi, s * a.lo};
+}
+template <class R> __device__ inline IvR<R> iadd(IvR<R> a, IvR<R> b) { return {a.lo + b.lo, a.hi + b.hi}; }
+
+// The normalized normal's affine form in tangent space over one cell:
+// centre mc, symbol coefficients ma, mb, remainder mk per axis.
+template <class R> struct MForm
+{
+    R c[3], a[3], b[3], k[3];
+};
+
+template <class R> struct AF
+{
+    R c, a, b, k;
+};
+template <class R> __device__ inline R af_dev(const AF<R> &f) { return fabs(f.a) + fabs(f.b) + f.k; }
+// Product (affine_form.cpp:37-47) with the rounding room of the float path.
+template <class R> __device__ inline AF<R> af_mul(const AF<R> &x, const AF<R> &r)
+{
+    R u = af_dev(x), v = af_dev(r);
+    AF<R> o;
+    o.c = x.c * r.c;
+    o.a = x.c * r.a + r.c * x.a;
+    o.b = x.c * r.b + r.c * x.b;
+    o.k = fabs(r.c) * x.k + fabs(x.c) * r.k + u * v;
+    o.k += pad_of<R>(fabs(o.c) + fabs(o.a) + fabs(o.b) + o.k);
+    return o;
+}
+
+// AffineVec3::normalize (affine_form.cpp:78-95: sq_length, Min-Range
+// rsqrt, product) with the rounding room of the float path.
+template <class R> __device__ void affine_normalize(AF<R> N[3])
+{
+    R dev[3], cc = 0, cu = 0, cv = 0, ck = 0, dd = 0;
+    for (int q = 0; q < 3; ++q) {
+        dev[q] = fabs(N[q].a) + fabs(N[q].b) + N[q].k;
+        cc += N[q].c * N[q].c;
+        cu += N[q].c * N[q].a;
+        cv += N[q].c * N[q].b;
+        ck += N[q].c * N[q].k;
+        dd += dev[q] * dev[q];
+    }
+    R offset = (R)0.5 * dd;
+    AF<R> l{cc + offset, (R)2 * cu, (R)2 * cv, (R)2 * fabs(ck) + offset};
+    l.k += pad_of<R>(cc + fabs(l.a) + fabs(l.b) + l.k);
+    R lo = l.c - af_dev(l), hi = l.c + af_dev(l);
+    if (lo <= (R)0) {
+        for (int q = 0; q < 3; ++q)
+            N[q] = AF<R>{0, 0, 0, 1};
+        return;
+    }
+    R glo = (R)1 / sqrt(lo), ghi = (R)1 / sqrt(hi);
+    R alpha = (R)-0.5 * ghi * ghi * ghi;
+    R beta = (R)0.5 * (glo + ghi - alpha * (lo + hi));
+    R delta = (R)0.5 * fabs(glo - ghi - alpha * (lo - hi));
+    delta += pad_of<R>(glo + ghi + fabs(alpha) * (lo + hi));
+    AF<R> rs{alpha * l.c + beta, alpha * l.a, alpha * l.b, fabs(alpha) * l.k + delta};
+    rs.k += pad_of<R>(fabs(alpha * l.c) + fabs(beta) + fabs(rs.a) + fabs(rs.b));
+    for (int q = 0; q < 3; ++q)
+        N[q] = af_mul(rs, N[q]);
+}
+
+// node_bound's normal form normalized (node_bounds.cpp:65-68, 100-103)
+// over the rectangle (cx, cy) +- (hu, hv), mapped to tangent space. The
+// general path, for a non-constant vertex normal.
+template <class R> __device__ void normal_mform(const TriRecord &T, R cx, R cy, R hu, R hv, MForm<R> &out)
+{
+    AF<R> N[3];
+    for (int q = 0; q < 3; ++q) {
+        N[q].c = (R)T.m0[q] + cx * (R)T.Mu[q] + cy * (R)T.Mv[q];
+        N[q].a = hu * (R)T.Mu[q];
+        N[q].b = hv * (R)T.Mv[q];
+        N[q].k = pad_of<R>(fabs((R)T.m0[q]) + fabs(cx * (R)T.Mu[q]) + fabs(cy * (R)T.Mv[q]));
+    }
+    affine_normalize<R>(N);
+    // to_tangent * N (affine_form.cpp:100-106): rows of T.
+    for (int r = 0; r < 3; ++r) {
+        R c = 0, a = 0, b = 0, k = 0, mag = 0;
+        for (int q = 0; q < 3; ++q) {
+            R t = (R)T.T[3 * r + q];
+            c += t * N[q].c;
+            a += t * N[q].a;
+            b += t * N[q].b;
+            k += fabs(t) * N[q].k;
+            mag += fabs(t) * (fabs(N[q].c) + fabs(N[q].a) + fabs(N[q].b));
+        }
+        out.c[r] = c;
+        out.a[r] = a;
+        out.b[r] = b;
+        out.k[r] = k + pad_of<R>(mag);
+    }
+}
+
+// Per-walk state of one lane.
+template <class R> struct Walk
+{
+    R o[3], d[3], inv[3]; // the ray in tangent space
+    R omag, dmag;
+    int si, sj;
+    int level, i, j;    // current cell
+    int elevel, ei, ej; // the cell that ends the current root's subtree
+    int r;              // current root
+};
+
+__device__ inline void cell_next(int &level, int &i, int &j, int si, int sj, int root_level)
+{
+    // cell_next (displaced_intersector.cpp:99-121); floor_div2 is the
+    // arithmetic shift, floor_mod2 the low bit.
+    while (true) {
+        int code = 2 * ((i + si) & 1) + ((j + sj) & 1);
+        if (code == 1) {
+            j += sj ? 1 : -1;
+            i += si ? -1 : 1;
+            return;
+        }
+        if (code == 3) {
+            ++level;
+            i >>= 1;
+            j >>= 1;
+            if (level > root_level)
+                return;
+            continue;
+        }
+        j += sj ? -1 : 1;
+        return;
+    }
+}
+
+// classify_square (uv_clip.cpp:27-44), Outside only, with rounding room so
+// that a cell the double test keeps is kept.
+template <class R> __device__ inline bool square_outside(const TriRecord &T, R cx, R cy, R s)
+{
+    R m = pad_of<R>((R)1 + fabs(cx) + fabs(cy) + s);
+    if (cx + s <= (R)T.dlo[0] - m || cx - s >= (R)T.dhi[0] + m || cy + s <= (R)T.dlo[1] - m ||
+        cy - s >= (R)T.dhi[1] + m)
+        return true;
+    for (int k = 0; k < 3; ++k) {
+        R nx = (R)T.dn[2 * k], ny = (R)T.dn[2 * k + 1];
+        R proj = nx * cx + ny * cy;
+        R ext = (fabs(nx) + fabs(ny)) * s;
+        R mk = pad_of<R>(fabs(nx * cx) + fabs(ny * cy) + ext + fabs((R)T.dd[k]));
+        if (proj + ext <= (R)T.dd[k] - mk)
+            return true;
+    }
+    return false;
+}
+template <> __device__ inline bool square_outside<double>(const TriRecord &T, double cx, double cy, double s)
+{
+    double m = pad_of<double>(1.0 + fabs(cx) + fabs(cy) + s);
+    if (cx + s <= T.dlo_d[0] - m || cx - s >= T.dhi_d[0] + m || cy + s <= T.dlo_d[1] - m || cy - s >= T.dhi_d[1] + m)
+        return true;
+    for (int k = 0; k < 3; ++k) {
+        double nx = T.dn_d[2 * k], ny = T.dn_d[2 * k + 1];
+        double proj = nx * cx + ny * cy;
+        double ext = (fabs(nx) + fabs(ny)) * s;
+        double mk = pad_of<double>(fabs(nx * cx) + fabs(ny * cy) + ext + fabs(T.dd_d[k]));
+        if (proj + ext <= T.dd_d[k] - mk)
+            return true;
+    }
+    return false;
+}
+
+// UvTriangle::inside (uv_clip.h:14-20).
+template <class R> __device__ inline bool domain_inside(const TriRecord &T, R s, R t)
+{
+    for (int k = 0; k < 3; ++k)
+        if ((R)T.dn_d[2 * k] * s + (R)T.dn_d[2 * k + 1] * t < (R)T.dd_d[k])
+            return false;
+    return true;
+}
+
+__device__ inline float2 fetch_mm(const PyramidView &P, int level, int i, int j)
+{
+    if (level >= P.n_levels)
+        return make_float2(P.root_hmin, P.root_hmax);
+    uint32_t m = (uint32_t)P.n_leaf >> level;
+    uint32_t idx = level_base(m) + morton2((uint32_t)i & (m - 1), (uint32_t)j & (m - 1));
+    return P.mm[idx];
+}
+__device__ inline float4 fetch_tp(const PyramidView &P, int level, int i, int j)
+{
+    if (level >= P.n_levels) {
+        // Above the root: the root's min-max node (taylor_pyramid.cpp:261-266).
+        return make_float4(0.5f * (P.root_hmin + P.root_hmax), 0.f, 0.f, 0.5f * (P.root_hmax - P.root_hmin));
+    }
+    uint32_t m = (uint32_t)P.n_leaf >> level;
+    uint32_t idx = level_base(m) + morton2((uint32_t)i & (m - 1), (uint32_t)j & (m - 1));
+    return P.tp[idx];
+}
+
+// node_bound + node_interval (node_bounds.cpp:83-109, 111-211) for the
+// current cell of a walk. Returns whether the ray can meet the surface
+// over the cell within [t_min, t_max].
+template <class R>
+__device__ bool node_test(const TraceParams &p, const TriRecord &T, const Walk<R> &wk, R t_min, R t_max)
+{
+    const PyramidView &P = p.pyr;
+    int level = wk.level;
+    R w = ldexp((R)1, level) / (R)P.n_leaf;
+    R clo_x = (R)wk.i * w, clo_y = (R)wk.j * w;
+    // The cell clipped to the triangle's texture bounding box (node_bounds.cpp:94-99).
+    R rlo_x = fmax(clo_x, (R)T.dlo[0]), rlo_y = fmax(clo_y, (R)T.dlo[1]);
+    R rhi_x = fmin(clo_x + w, (R)T.dhi[0]), rhi_y = fmin(clo_y + w, (R)T.dhi[1]);
+    if (sizeof(R) == 8) {
+        rlo_x = fmax(clo_x, (R)T.dlo_d[0]);
+        rlo_y = fmax(clo_y, (R)T.dlo_d[1]);
+        rhi_x = fmin(clo_x + w, (R)T.dhi_d[0]);
+        rhi_y = fmin(clo_y + w, (R)T.dhi_d[1]);
+    }
+    R cx = (R)0.5 * (rlo_x + rhi_x), cy = (R)0.5 * (rlo_y + rhi_y);
+    R hx = fmax((R)0.5 * (rhi_x - rlo_x), (R)0), hy = fmax((R)0.5 * (rhi_y - rlo_y), (R)0);
+
+    MForm<R> m;
+    if (T.const_normal) {
+        for (int q = 0; q < 3; ++q) {
+            m.c[q] = sizeof(R) == 8 ? (R)T.mc_d[q] : (R)T.mc[q];
+            m.a[q] = 0;
+            m.b[q] = 0;
+            m.k[q] = sizeof(R) == 8 ? (R)T.mk_d[q] : (R)T.mk[q];
+        }
+    } else {
+        normal_mform<R>(T, cx, cy, hx, hy, m);
+    }
+
+    float2 mm = fetch_mm(P, level, wk.i, wk.j);
+    R hc = (R)0.5 * ((R)mm.x + (R)mm.y), hr = (R)0.5 * ((R)mm.y - (R)mm.x);
+    // box = hull(p + h m): p = (cx +- hx, cy +- hy, 0), h = hc +- hr.
+    R blo[3], bhi[3];
+    const R pc[3] = {cx, cy, (R)0}, pa[3] = {hx, (R)0, (R)0}, pb[3] = {(R)0, hy, (R)0};
+    for (int q = 0; q < 3; ++q) {
+        R dev_m = fabs(m.a[q]) + fabs(m.b[q]) + m.k[q];
+        R c = pc[q] + hc * m.c[q];
+        R a = pa[q] + hc * m.a[q];
+        R b = pb[q] + hc * m.b[q];
+        R k = fabs(m.c[q]) * hr + fabs(hc) * m.k[q] + hr * dev_m;
+        R e = fabs(a) + fabs(b) + k;
+        e += pad_of<R>(fabs(c) + e);
+        blo[q] = c - e;
+        bhi[q] = c + e;
+    }
+
+    bool read_slab = level <= p.slab_max_level;
+    bool want_box = p.bound == 0 || p.bound == 2 || !read_slab;
+    bool want_slab = p.bound != 0 && read_slab;
+
+    R b0 = t_min, b1 = t_max;
+    if (want_box) {
+        if (!clip_axis<R>(wk.o[0], wk.d[0], wk.inv[0], blo[0], bhi[0], b0, b1) ||
+            !clip_axis<R>(wk.o[1], wk.d[1], wk.inv[1], blo[1], bhi[1], b0, b1) ||
+            !clip_axis<R>(wk.o[2], wk.d[2], wk.inv[2], blo[2], bhi[2], b0, b1) || !close_interval<R>(b0, b1))
+            return false;
+        if (!want_slab)
+            return true;
+    }
+
+    // slab_interval (node_bounds.cpp:137-179): the column first.
+    R t0 = t_min, t1 = t_max;
+    if (!clip_axis<R>(wk.o[0], wk.d[0], wk.inv[0], blo[0], bhi[0], t0, t1) ||
+        !clip_axis<R>(wk.o[1], wk.d[1], wk.inv[1], blo[1], bhi[1], t0, t1) || !close_interval<R>(t0, t1))
+        return false;
+    R anchor[3];
+    for (int q = 0; q < 3; ++q)
+        anchor[q] = wk.o[q] + t0 * wk.d[q];
+    R span = t1 - t0;
+    float4 tp = fetch_tp(P, level, wk.i, wk.j);
+    R h0 = tp.x, gu = tp.y, gv = tp.z, rr = tp.w;
+    if (level >= P.n_levels) {
+        gu = 0;
+        gv = 0;
+    }
+    R ucx = ((R)wk.i + (R)0.5) * w, ucy = ((R)wk.j + (R)0.5) * w; // nb.centre
+    IvR<R> mx{m.c[0] - (fabs(m.a[0]) + fabs(m.b[0]) + m.k[0]), m.c[0] + (fabs(m.a[0]) + fabs(m.b[0]) + m.k[0])};
+    IvR<R> my{m.c[1] - (fabs(m.a[1]) + fabs(m.b[1]) + m.k[1]), m.c[1] + (fabs(m.a[1]) + fabs(m.b[1]) + m.k[1])};
+    IvR<R> mz{m.c[2] - (fabs(m.a[2]) + fabs(m.b[2]) + m.k[2]), m.c[2] + (fabs(m.a[2]) + fabs(m.b[2]) + m.k[2])};
+    IvR<R> A = iadd(iadd(IvR<R>{(R)1, (R)1}, smul(gu, mx)), smul(gv, my));
+    IvR<R> B = smul(-gu, mz), C = smul(-gv, mz);
+    R dconst = -(h0 - gu * ucx - gv * ucy);
+    IvR<R> D = smul(dconst, mz);
+    R Rr = rr * mz.mag();
+    IvR<R> F0 = iadd(iadd(iadd(smul(anchor[2], A), smul(anchor[0], B)), smul(anchor[1], C)), D);
+    IvR<R> F1 = iadd(iadd(smul(wk.d[2], A), smul(wk.d[0], B)), smul(wk.d[1], C));
+    // Rounding room of everything above, folded into the remainder R.
+    R magA = (R)1 + fabs(gu) * mx.mag() + fabs(gv) * my.mag(), magB = fabs(gu) * mz.mag(),
+      magC = fabs(gv) * mz.mag();
+    R magD = (fabs(h0) + fabs(gu * ucx) + fabs(gv * ucy)) * mz.mag();
+    R err = (fabs(anchor[2]) + span * fabs(wk.d[2])) * magA + (fabs(anchor[0]) + span * fabs(wk.d[0])) * magB +
+            (fabs(anchor[1]) + span * fabs(wk.d[1])) * magC + magD +
+            (wk.omag + fabs(t0) * wk.dmag) * (magA + magB + magC) + Rr + F0.mag();
+    Rr += pad_of<R>(err);
+    R s0 = 0, s1 = span;
+    if (F1.lo > (R)0)
+        s1 = fmin(s1, (Rr - F0.lo) / F1.lo);
+    else if (F1.lo < (R)0)
+        s0 = fmax(s0, (Rr - F0.lo) / F1.lo);
+    else if (F0.lo > Rr)
+        return false;
+    if (F1.hi > (R)0)
+        s0 = fmax(s0, (-Rr - F0.hi) / F1.hi);
+    else if (F1.hi < (R)0)
+        s1 = fmin(s1, (-Rr - F0.hi) / F1.hi);
+    else if (F0.hi < -Rr)
+        return false;
+    s0 -= pad_of<R>(fabs(s0));
+    s1 += pad_of<R>(fabs(s1));
+    if (s0 > s1 + tolerance<R>(fabs(t0) + span))
+        return false;
+    if (s0 > s1) {
+        R x = s0;
+        s0 = s1;
+        s1 = x;
+    }
+    R st1 = t0 + s1, st0 = t0 + s0;
+    if (!want_box)
+        return true;
+    // both_interval (node_bounds.cpp:181-197).
+    R u0 = fmax(b0, st0), u1 = fmin(b1, st1);
+    return close_interval<R>(u0, u1);
+}
+
+// ------------------------------------------------------------------ leaf
+
+// Per-leaf constants of the certified Newton leaf, in texel-local
+// coordinates (a, b) in [0, 1]^2: S(a, b) - o = (P0 - o) + a E1 + b E2 +
+// h(a, b) N(a, b), E1 = w e1, E2 = w e2, P0 = P(i w, j w).
+template <class R> struct Leaf
+{
+    R c1, c2, cn;                     // d1, d2, dn . (P0 - o)
+    R p1a, p1b, p2a, p2b, pna, pnb;   // d1, d2, dn . E1, E2
+    R h00, ha, hb, tw;                // h = h00 + ha a + hb b + tw a b
+    R n1, n2, nn;                     // d1, d2, dn . N (constant normal)
+    R a1, a2, an;                     // sum |d_x| |N_x|
+    R k1, k2, kn;                     // sum |d_x| N remainder_x
+    R E1[3], E2[3], N[3];             // object-space vectors for the normal
+    R d1[3], d2[3], dn[3];            // ray basis (general normal path)
+    R M0[3], wMu[3], wMv[3];          // interpolated normal over the texel (general path)
+    R hmax, magF, tol2, d_len;
+    R slo, tlo, w;
+    int i, j;
+};
+
+// plane_basis (displaced_intersector.cpp:22-27) in double.
+__device__ inline void plane_basis(const double n[3], double d1[3], double d2[3])
+{
+    double a[3] = {fabs(n[0]) < 0.9 ? 1.0 : 0.0, fabs(n[0]) < 0.9 ? 0.0 : 1.0, 0.0};
+    double c[3] = {a[1] * n[2] - a[2] * n[1], a[2] * n[0] - a[0] * n[2], a[0] * n[1] - a[1] * n[0]};
+    double len = sqrt(c[0] * c[0] + c[1] * c[1] + c[2] * c[2]);
+    for (int q = 0; q < 3; ++q)
+        d1[q] = c[q] / len;
+    d2[0] = n[1] * d1[2] - n[2] * d1[1];
+    d2[1] = n[2] * d1[0] - n[0] * d1[2];
+    d2[2] = n[0] * d1[1] - n[1] * d1[0];
+}
+
+__device__ inline float node_value(const PyramidView &P, int i, int j)
+{
+    // HeightGrid::h at a node (displaced_surface.cpp:12-39): node (i, j) of
+    // the closed grid, the texture wrapping (or clamping) outside the tile.
+    int n = P.n_leaf;
+    if (P.repeat) {
+        i &= n - 1;
+        j &= n - 1;
+        // the closed grid's last row and column repeat the first, so the
+        // wrapped index n never occurs: nodes (n, .) equal (0, .).
+    } else {
+        i = min(max(i, 0), n);
+        j = min(max(j, 0), n);
+    }
+    return P.nodes[(size_t)j * (n + 1) + i];
+}
+
+template <class R>
+__device__ void leaf_setup(const TraceParams &p, const TriRecord &T, const double o[3], const double d[3], int i,
+                           int j, Leaf<R> &L)
+{
+    const PyramidView &P = p.pyr;
+    double w = 1.0 / P.n_leaf;
+    double slo = i * w, tlo = j * w;
+    double P0[3], rel[3];
+    for (int q = 0; q < 3; ++q) {
+        P0[q] = T.q0[q] + slo * T.e1[q] + tlo * T.e2[q];
+        rel[q] = P0[q] - o[q];
+    }
+    double d_len = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+    double dn[3] = {d[0] / d_len, d[1] / d_len, d[2] / d_len}, d1[3], d2[3];
+    plane_basis(dn, d1, d2);
+    auto dot = [](const double *a, const double *b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; };
+    double E1[3] = {w * T.e1[0], w * T.e1[1], w * T.e1[2]}, E2[3] = {w * T.e2[0], w * T.e2[1], w * T.e2[2]};
+    L.c1 = (R)dot(d1, rel);
+    L.c2 = (R)dot(d2, rel);
+    L.cn = (R)dot(dn, rel);
+    L.p1a = (R)dot(d1, E1);
+    L.p1b = (R)dot(d1, E2);
+    L.p2a = (R)dot(d2, E1);
+    L.p2b = (R)dot(d2, E2);
+    L.pna = (R)dot(dn, E1);
+    L.pnb = (R)dot(dn, E2);
+    for (int q = 0; q < 3; ++q) {
+        L.E1[q] = (R)E1[q];
+        L.E2[q] = (R)E2[q];
+        L.d1[q] = (R)d1[q];
+        L.d2[q] = (R)d2[q];
+        L.dn[q] = (R)dn[q];
+    }
+    // Corner heights h = scale * value + offset (HeightGrid::h at the corners).
+    R sc = sizeof(R) == 8 ? (R)P.scale_d : (R)P.scale, of = sizeof(R) == 8 ? (R)P.offset_d : (R)P.offset;
+    R h00 = sc * (R)node_value(P, i, j) + of, h10 = sc * (R)node_value(P, i + 1, j) + of;
+    R h01 = sc * (R)node_value(P, i, j + 1) + of, h11 = sc * (R)node_value(P, i + 1, j + 1) + of;
+    L.h00 = h00;
+    L.ha = h10 - h00;
+    L.hb = h01 - h00;
+    L.tw = ((h11 - h10) - h01) + h00;
+    L.hmax = fmax(fmax(fabs(h00), fabs(h10)), fmax(fabs(h01), fabs(h11)));
+    if (T.const_normal) {
+        double N[3] = {T.nhat_d[0], T.nhat_d[1], T.nhat_d[2]};
+        double nk = T.nhat_kd;
+        L.n1 = (R)dot(d1, N);
+        L.n2 = (R)dot(d2, N);
+        L.nn = (R)dot(dn, N);
+        L.a1 = (R)(fabs(d1[0] * N[0]) + fabs(d1[1] * N[1]) + fabs(d1[2] * N[2]));
+        L.a2 = (R)(fabs(d2[0] * N[0]) + fabs(d2[1] * N[1]) + fabs(d2[2] * N[2]));
+        L.an = (R)(fabs(dn[0] * N[0]) + fabs(dn[1] * N[1]) + fabs(dn[2] * N[2]));
+        L.k1 = (R)(nk * (fabs(d1[0]) + fabs(d1[1]) + fabs(d1[2])));
+        L.k2 = (R)(nk * (fabs(d2[0]) + fabs(d2[1]) + fabs(d2[2])));
+        L.kn = (R)(nk * (fabs(dn[0]) + fabs(dn[1]) + fabs(dn[2])));
+        for (int q = 0; q < 3; ++q)
+            L.N[q] = (R)N[q];
+    } else {
+        for (int q = 0; q < 3; ++q) {
+            L.M0[q] = (R)(T.m0[q] + slo * T.Mu[q] + tlo * T.Mv[q]);
+            L.wMu[q] = (R)(w * T.Mu[q]);
+            L.wMv[q] = (R)(w * T.Mv[q]);
+        }
+    }
+    L.d_len = (R)d_len;
+    L.slo = (R)slo;
+    L.tlo = (R)tlo;
+    L.w = (R)w;
+    L.i = i;
+    L.j = j;
+    R magF = fmax(fabs(L.c1), fabs(L.c2)) + fabs(L.p1a) + fabs(L.p1b) + fabs(L.p2a) + fabs(L.p2b) + L.hmax;
+    L.magF = magF;
+    R tol = fmax((R)T.residual_tolerance, (R)KNEWTON * Num<R>::eps * magF);
+    L.tol2 = tol * tol;
+}
+
+// The normal N(a, b), its a- and b-derivatives, over the texel (general
+// normal path): N = M / |M|, N_a = (wMu - N (N . wMu)) / |M|
+// (normal_frame_at, displaced_surface.cpp:99-108, in local units).
+template <class R> __device__ inline void normal_at(const Leaf<R> &L, R a, R b, R N[3], R Na[3], R Nb[3])
+{
+    R M[3];
+    for (int q = 0; q < 3; ++q)
+        M[q] = L.M0[q] + a * L.wMu[q] + b * L.wMv[q];
+    R len = sqrt(M[0] * M[0] + M[1] * M[1] + M[2] * M[2]);
+    for (int q = 0; q < 3; ++q)
+        N[q] = M[q] / len;
+    R nu = N[0] * L.wMu[0] + N[1] * L.wMu[1] + N[2] * L.wMu[2];
+    R nv = N[0] * L.wMv[0] + N[1] * L.wMv[1] + N[2] * L.wMv[2];
+    for (int q = 0; q < 3; ++q) {
+        Na[q] = (L.wMu[q] - N[q] * nu) / len;
+        Nb[q] = (L.wMv[q] - N[q] * nv) / len;
+    }
+}
+
+// The normalized normal over the whole texel as an affine form in object
+// space (intersect_newton, displaced_intersector.cpp:250-251): centre at the
+// texel centre, symbol coefficients w Mu / 2 and w Mv / 2.
+template <class R> __device__ void normal_texel_form(const Leaf<R> &L, MForm<R> &nf)
+{
+    AF<R> N[3];
+    for (int q = 0; q < 3; ++q) {
+        N[q].c = L.M0[q] + (R)0.5 * L.wMu[q] + (R)0.5 * L.wMv[q];
+        N[q].a = (R)0.5 * L.wMu[q];
+        N[q].b = (R)0.5 * L.wMv[q];
+        N[q].k = pad_of<R>(fabs(L.M0[q]) + fabs(L.wMu[q]) + fabs(L.wMv[q]));
+    }
+    affine_normalize<R>(N);
+    for (int q = 0; q < 3; ++q) {
+        nf.c[q] = N[q].c;
+        nf.a[q] = N[q].a;
+        nf.b[q] = N[q].b;
+        nf.k[q] = N[q].k;
+    }
+}
+
+// sub_square_bound (displaced_intersector.cpp:266-290) on the sub-square
+// with centre (a, b) and half-width rho (texel units).
+template <class R>
+__device__ bool sub_square_bound(const TriRecord &T, const Leaf<R> &L, R a, R b, R rho, R t_min, R t_max, R &t_lo)
+{
+    if (square_outside<R>(T, L.slo + L.w * a, L.tlo + L.w * b, L.w * rho))
+        return false;
+    // h over the sub-square: affine up to the twist (rho eu)(rho ev).
+    R hc = ((L.h00 + L.ha * a) + L.hb * b) + L.tw * a * b;
+    R hA = (L.ha + L.tw * b) * rho, hB = (L.hb + L.tw * a) * rho, hk = fabs(L.tw) * rho * rho;
+    R hmag = fabs(hc) + fabs(hA) + fabs(hB) + hk;
+    R F1c, F1a, F1b, F1k, F2c, F2a, F2b, F2k, Gc, Ga, Gb, Gk;
+    if (T.const_normal) {
+        F1c = L.c1 + a * L.p1a + b * L.p1b + hc * L.n1;
+        F1a = rho * L.p1a + hA * L.n1;
+        F1b = rho * L.p1b + hB * L.n1;
+        F1k = hk * L.a1 + hmag * L.k1;
+        F2c = L.c2 + a * L.p2a + b * L.p2b + hc * L.n2;
+        F2a = rho * L.p2a + hA * L.n2;
+        F2b = rho * L.p2b + hB * L.n2;
+        F2k = hk * L.a2 + hmag * L.k2;
+        Gc = L.cn + a * L.pna + b * L.pnb + hc * L.nn;
+        Ga = rho * L.pna + hA * L.nn;
+        Gb = rho * L.pnb + hB * L.nn;
+        Gk = hk * L.an + hmag * L.kn;
+    } else {
+        // N_hat over the texel, restricted to the sub-square
+        // (displaced_intersector.cpp:72-81, 250-251).
+        MForm<R> nf;
+        normal_texel_form(L, nf);
+        AF<R> h{hc, hA, hB, hk};
+        AF<R> S[3];
+        R alpha = (R)2 * a - (R)1, beta = (R)2 * b - (R)1, r2 = (R)2 * rho;
+        for (int q = 0; q < 3; ++q) {
+            AF<R> nq{nf.c[q] + nf.a[q] * alpha + nf.b[q] * beta, nf.a[q] * r2, nf.b[q] * r2, nf.k[q]};
+            AF<R> hn = af_mul(h, nq);
+            S[q] = AF<R>{a * L.E1[q] + b * L.E2[q] + hn.c, rho * L.E1[q] + hn.a, rho * L.E2[q] + hn.b, hn.k};
+        }
+        auto proj = [&](const R *dv, R base, R &c, R &aa, R &bb, R &kk) {
+            c = base + dv[0] * S[0].c + dv[1] * S[1].c + dv[2] * S[2].c;
+            aa = dv[0] * S[0].a + dv[1] * S[1].a + dv[2] * S[2].a;
+            bb = dv[0] * S[0].b + dv[1] * S[1].b + dv[2] * S[2].b;
+            kk = fabs(dv[0]) * S[0].k + fabs(dv[1]) * S[1].k + fabs(dv[2]) * S[2].k;
+        };
+        proj(L.d1, L.c1, F1c, F1a, F1b, F1k);
+        proj(L.d2, L.c2, F2c, F2a, F2b, F2k);
+        proj(L.dn, L.cn, Gc, Ga, Gb, Gk);
+    }
+    R local = (fabs(a) + rho) * (fabs(L.p1a) + fabs(L.p1b) + fabs(L.p2a) + fabs(L.p2b) + fabs(L.pna) + fabs(L.pnb)) +
+              hmag;
+    R e1 = fabs(F1a) + fabs(F1b) + F1k + pad_of<R>(fabs(L.c1) + local);
+    R e2 = fabs(F2a) + fabs(F2b) + F2k + pad_of<R>(fabs(L.c2) + local);
+    if (!(F1c - e1 <= (R)0 && (R)0 <= F1c + e1) || !(F2c - e2 <= (R)0 && (R)0 <= F2c + e2))
+        return false;
+    R eg = fabs(Ga) + fabs(Gb) + Gk + pad_of<R>(fabs(L.cn) + local);
+    t_lo = (Gc - eg) / L.d_len;
+    t_lo -= pad_of<R>(fabs(t_lo));
+    R t_hi = (Gc + eg) / L.d_len;
+    t_hi += pad_of<R>(fabs(t_hi));
+    return t_lo < t_max && t_hi > t_min;
+}
+
+// newton_from (displaced_intersector.cpp:325-388) in texel-local
+// coordinates. On success writes the root (a, b) and its t.
+template <class R>
+__device__ bool newton_from(const TriRecord &T, const Leaf<R> &L, R a, R b, R t_min, R t_max, R &out_a, R &out_b,
+                            R &out_t, int &iters)
+{
+    R prev_err2 = (R)INFINITY;
+    int err_streak = 0, behind_streak = 0, invalid_streak = 0;
+    for (int iter = 0; iter < MAX_ITERS; ++iter) {
+        ++iters;
+        R h = ((L.h00 + L.ha * a) + L.hb * b) + L.tw * a * b;
+        R hda = L.ha + L.tw * b, hdb = L.hb + L.tw * a;
+        R F1, F2, along, J00, J01, J10, J11;
+        if (T.const_normal) {
+            F1 = L.c1 + a * L.p1a + b * L.p1b + h * L.n1;
+            F2 = L.c2 + a * L.p2a + b * L.p2b + h * L.n2;
+            along = L.cn + a * L.pna + b * L.pnb + h * L.nn;
+            J00 = L.p1a + hda * L.n1;
+            J01 = L.p1b + hdb * L.n1;
+            J10 = L.p2a + hda * L.n2;
+            J11 = L.p2b + hdb * L.n2;
+        } else {
+            R N[3], Na[3], Nb[3];
+            normal_at(L, a, b, N, Na, Nb);
+            R Sl[3], Sa[3], Sb[3];
+            for (int q = 0; q < 3; ++q) {
+                Sl[q] = a * L.E1[q] + b * L.E2[q] + h * N[q];
+                Sa[q] = L.E1[q] + hda * N[q] + h * Na[q];
+                Sb[q] = L.E2[q] + hdb * N[q] + h * Nb[q];
+            }
+            auto dt = [](const R *x, const R *y) { return x[0] * y[0] + x[1] * y[1] + x[2] * y[2]; };
+            F1 = L.c1 + dt(L.d1, Sl);
+            F2 = L.c2 + dt(L.d2, Sl);
+            along = L.cn + dt(L.dn, Sl);
+            J00 = dt(L.d1, Sa);
+            J01 = dt(L.d1, Sb);
+            J10 = dt(L.d2, Sa);
+            J11 = dt(L.d2, Sb);
+        }
+        R err2 = F1 * F1 + F2 * F2;
+        err_streak = err2 > prev_err2 ? err_streak + 1 : 0;
+        behind_streak = along < (R)0 ? behind_streak + 1 : 0;
+        if (err_streak >= 2 || behind_streak >= 2)
+            return false;
+        prev_err2 = err2;
+        if (err2 < L.tol2) {
+            if (along < (R)0 || !domain_inside<R>(T, L.slo + L.w * a, L.tlo + L.w * b))
+                return false;
+            R t_hit = along / L.d_len;
+            if (t_hit <= t_min || t_hit >= t_max)
+                return false;
+            out_a = a;
+            out_b = b;
+            out_t = t_hit;
+            return true;
+        }
+        R det = J00 * J11 - J01 * J10;
+        if (det == (R)0)
+            return false;
+        a -= (J11 * F1 - J01 * F2) / det;
+        b -= (-J10 * F1 + J00 * F2) / det;
+        bool outside = a < (R)0 || a > (R)1 || b < (R)0 || b > (R)1 ||
+                       !domain_inside<R>(T, L.slo + L.w * a, L.tlo + L.w * b);
+        if (outside) {
+            if (++invalid_streak >= 3)
+                return false;
+            a = fmin(fmax(a, (R)0), (R)1);
+            b = fmin(fmax(b, (R)0), (R)1);
+        } else {
+            invalid_streak = 0;
+        }
+    }
+    return false;
+}
+
+template <class R> struct StackSlot;
+template <> struct StackSlot<float>
+{
+    using T = float;
+};
+template <> struct StackSlot<double>
+{
+    using T = double;
+};
+
+struct LaneStats
+{
+    int tri, node, leaf, sub, runs, iters;
+};
+
+// search_leaf (displaced_intersector.cpp:292-323) as a depth-first search
+// over an explicit stack; entries are (depth, qa, qb) packed in 16 bits and
+// the sub-square's lower bound on t. Children are pushed farthest first,
+// so they pop nearest first, as the recursion visits them.
+template <class R>
+__device__ bool search_leaf(const TriRecord &T, const Leaf<R> &L, R t_min, R &t_max, R &hit_a, R &hit_b,
+                            typename StackSlot<R>::T *s_t, uint16_t *s_code, LaneStats &st)
+{
+    const int lane = threadIdx.x;
+    int sp = 0;
+    bool found = false;
+    uint32_t code = 0; // depth 0, the whole texel
+    while (true) {
+        int depth = code >> 10, qa = (code >> 5) & 31, qb = code & 31;
+        if (depth == SUBDIV_DEPTH) {
+            ++st.runs;
+            R scale = (R)1 / (R)(1 << SUBDIV_DEPTH);
+            R ra, rb, rt;
+            if (newton_from<R>(T, L, ((R)qa + (R)0.5) * scale, ((R)qb + (R)0.5) * scale, t_min, t_max, ra, rb, rt,
+                               st.iters)) {
+                hit_a = ra;
+                hit_b = rb;
+                t_max = rt;
+                found = true;
+            }
+        } else {
+            int cd = depth + 1;
+            R inv = (R)1 / (R)(1 << cd);
+            R rho = (R)0.5 * inv;
+            R key[4];
+            uint32_t cc[4];
+            for (int q = 0; q < 4; ++q) {
+                int ca = 2 * qa + (q & 1), cb = 2 * qb + (q >> 1);
+                cc[q] = ((uint32_t)cd << 10) | ((uint32_t)ca << 5) | (uint32_t)cb;
+                ++st.sub;
+                R tl;
+                bool keep = sub_square_bound<R>(T, L, ((R)ca + (R)0.5) * inv, ((R)cb + (R)0.5) * inv, rho, t_min,
+                                               t_max, tl);
+                key[q] = keep ? tl : (R)INFINITY;
+            }
+            // Sort the four by key (a network that keeps equal keys in q order).
+#define DMAP_CSWAP(x, y)                                                                                              \
+    if (key[y] < key[x]) {                                                                                            \
+        R tk = key[x];                                                                                                \
+        key[x] = key[y];                                                                                              \
+        key[y] = tk;                                                                                                  \
+        uint32_t tc = cc[x];                                                                                          \
+        cc[x] = cc[y];                                                                                                \
+        cc[y] = tc;                                                                                                   \
+    }
+            DMAP_CSWAP(0, 1)
+            DMAP_CSWAP(2, 3)
+            DMAP_CSWAP(0, 2)
+            DMAP_CSWAP(1, 3)
+            DMAP_CSWAP(1, 2)
+#undef DMAP_CSWAP
+            for (int q = 3; q >= 0; --q)
+                if (key[q] != (R)INFINITY) {
+                    s_t[sp * BLOCK + lane] = key[q];
+                    s_code[sp * BLOCK + lane] = (uint16_t)cc[q];
+                    ++sp;
+                }
+        }
+        // Pop the next sub-square that can still hold a nearer root.
+        bool have = false;
+        while (sp > 0) {
+            --sp;
+            R tl = s_t[sp * BLOCK + lane];
+            if (tl < t_max) {
+                code = s_code[sp * BLOCK + lane];
+                have = true;
+                break;
+            }
+        }
+        if (!have)
+            break;
+    }
+    return found;
+}
+
+// The two-triangle leaf (displaced_intersector.cpp:390-426, Moller-Trumbore
+// :30-49) with corners relative to the ray origin.
+template <class R>
+__device__ bool leaf_two_triangle(const TraceParams &p, const TriRecord &T, const Leaf<R> &L, const double o[3],
+                                  const double d[3], R t_min, R &t_max, R &hit_a, R &hit_b)
+{
+    // Corner positions S - o; rel = P0 - o is c = (c1, c2, cn) in the
+    // (d1, d2, dn) basis, rebuilt in object space.
+    R rel[3];
+    for (int q = 0; q < 3; ++q)
+        rel[q] = L.c1 * L.d1[q] + L.c2 * L.d2[q] + L.cn * L.dn[q];
+    R hc[4] = {L.h00, L.h00 + L.ha, L.h00 + L.hb, L.h00 + L.ha + L.hb + L.tw};
+    const R ca[4] = {0, 1, 0, 1}, cb[4] = {0, 0, 1, 1};
+    R S[4][3];
+    for (int c = 0; c < 4; ++c) {
+        R N[3], Na[3], Nb[3];
+        if (T.const_normal) {
+            for (int q = 0; q < 3; ++q)
+                N[q] = L.N[q];
+        } else {
+            normal_at(L, ca[c], cb[c], N, Na, Nb);
+        }
+        for (int q = 0; q < 3; ++q)
+            S[c][q] = rel[q] + ca[c] * L.E1[q] + cb[c] * L.E2[q] + hc[c] * N[q];
+    }
+    R dd[3] = {(R)d[0], (R)d[1], (R)d[2]};
+    // Faces (00, 10, 01) and (10, 11, 01) (tessellation_grid's split).
+    const int A[2] = {0, 1}, B[2] = {1, 3}, C[2] = {2, 2};
+    bool found = false;
+    for (int f = 0; f < 2; ++f) {
+        R e1[3], e2[3], pv[3], s[3], qv[3];
+        for (int q = 0; q < 3; ++q) {
+            e1[q] = S[B[f]][q] - S[A[f]][q];
+            e2[q] = S[C[f]][q] - S[A[f]][q];
+            s[q] = -S[A[f]][q];
+        }
+        pv[0] = dd[1] * e2[2] - dd[2] * e2[1];
+        pv[1] = dd[2] * e2[0] - dd[0] * e2[2];
+        pv[2] = dd[0] * e2[1] - dd[1] * e2[0];
+        R det = e1[0] * pv[0] + e1[1] * pv[1] + e1[2] * pv[2];
+        if (det == (R)0)
+            continue;
+        R inv = (R)1 / det;
+        R b1 = (s[0] * pv[0] + s[1] * pv[1] + s[2] * pv[2]) * inv;
+        if (b1 < (R)0 || b1 > (R)1)
+            continue;
+        qv[0] = s[1] * e1[2] - s[2] * e1[1];
+        qv[1] = s[2] * e1[0] - s[0] * e1[2];
+        qv[2] = s[0] * e1[1] - s[1] * e1[0];
+        R b2 = (dd[0] * qv[0] + dd[1] * qv[1] + dd[2] * qv[2]) * inv;
+        if (b2 < (R)0 || b1 + b2 > (R)1)
+            continue;
+        R t = (e2[0] * qv[0] + e2[1] * qv[1] + e2[2] * qv[2]) * inv;
+        if (!(t > t_min && t < t_max))
+            continue;
+        R la = (1 - b1 - b2) * ca[A[f]] + b1 * ca[B[f]] + b2 * ca[C[f]];
+        R lb = (1 - b1 - b2) * cb[A[f]] + b1 * cb[B[f]] + b2 * cb[C[f]];
+        if (!domain_inside<R>(T, L.slo + L.w * la, L.tlo + L.w * lb))
+            continue;
+        hit_a = la;
+        hit_b = lb;
+        t_max = t;
+        found = true;
+    }
+    return found;
+}
+
+// ------------------------------------------------------------------ kernel
+
+enum
+{
+    ST_NEXT = 0,
+    ST_WALK = 1,
+    ST_LEAF = 2,
+    ST_DONE = 3
+};
+
+template <class R, bool STATS>
+__global__ void __launch_bounds__(BLOCK) k_trace(TraceParams p)
+{
+    __shared__ typename StackSlot<R>::T s_t[STACK * BLOCK];
+    __shared__ uint16_t s_code[STACK * BLOCK];
+
+    int64_t rid = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+    bool valid = rid < p.n_rays;
+    double o[3] = {0, 0, 0}, d[3] = {0, 0, 1};
+    R t_min = 0, t_best = 0;
+    if (valid) {
+        o[0] = p.rays.ox[rid];
+        o[1] = p.rays.oy[rid];
+        o[2] = p.rays.oz[rid];
+        d[0] = p.rays.dx[rid];
+        d[1] = p.rays.dy[rid];
+        d[2] = p.rays.dz[rid];
+        t_min = (R)p.rays.tmin[rid];
+        t_best = (R)p.rays.tmax[rid];
+    }
+    const R o_f[3] = {(R)o[0], (R)o[1], (R)o[2]}, d_f[3] = {(R)d[0], (R)d[1], (R)d[2]};
+    R inv_f[3] = {(R)1 / d_f[0], (R)1 / d_f[1], (R)1 / d_f[2]};
+
+    int state = valid ? ST_NEXT : ST_DONE;
+    int k = p.prim_mask >= 0 ? p.prim_mask - 1 : -1;
+    int k_end = p.prim_mask >= 0 ? p.prim_mask + 1 : p.n_tri;
+    int hit_tri = -1, hit_i = 0, hit_j = 0;
+    R hit_a = 0, hit_b = 0;
+    LaneStats st{0, 0, 0, 0, 0, 0};
+    Walk<R> wk;
+    int root_level = 0, n_roots = 0;
+    const TriRecord *T = p.tris;
+
+    while (true) {
+        // Walk until a leaf is pending or the ray is done.
+        while (state == ST_NEXT || state == ST_WALK) {
+            if (state == ST_NEXT) {
+                // intersect_flat (displaced_intersector.cpp:578-599): the next
+                // triangle whose box the ray meets before the current hit.
+                ++k;
+                if (k >= k_end) {
+                    state = ST_DONE;
+                    break;
+                }
+                const float *bx = p.boxes + 6 * k;
+                R t0 = t_min, t1 = t_best;
+                if (!clip_axis<R>(o_f[0], d_f[0], inv_f[0], (R)bx[0], (R)bx[3], t0, t1) ||
+                    !clip_axis<R>(o_f[1], d_f[1], inv_f[1], (R)bx[1], (R)bx[4], t0, t1) ||
+                    !clip_axis<R>(o_f[2], d_f[2], inv_f[2], (R)bx[2], (R)bx[5], t0, t1) || !close_interval<R>(t0, t1))
+                    continue;
+                if (STATS)
+                    ++st.tri;
+                T = p.tris + k;
+                // The ray in tangent space (DisplacedIntersector::intersect :178).
+                double rel[3] = {o[0] - T->q0[0], o[1] - T->q0[1], o[2] - T->q0[2]};
+                R omag = 0, dmag = 0;
+                for (int r = 0; r < 3; ++r) {
+                    const double *row = T->T + 3 * r;
+                    wk.o[r] = (R)(row[0] * rel[0] + row[1] * rel[1] + row[2] * rel[2]);
+                    wk.d[r] = (R)(row[0] * d[0] + row[1] * d[1] + row[2] * d[2]);
+                    wk.inv[r] = (R)1 / wk.d[r];
+                    omag = fmax(omag, fabs(wk.o[r]));
+                    dmag = fmax(dmag, fabs(wk.d[r]));
+                }
+                wk.omag = omag;
+                wk.dmag = dmag;
+                wk.si = wk.d[0] < (R)0;
+                wk.sj = wk.d[1] < (R)0;
+                root_level = T->root_level;
+                n_roots = T->n_roots;
+                wk.r = 0;
+                wk.level = root_level;
+                wk.i = T->root_i[0];
+                wk.j = T->root_j[0];
+                wk.elevel = root_level;
+                wk.ei = wk.i;
+                wk.ej = wk.j;
+                cell_next(wk.elevel, wk.ei, wk.ej, wk.si, wk.sj, root_level);
+                state = ST_WALK;
+                continue;
+            }
+            // ST_WALK
+            if (wk.level == wk.elevel && wk.i == wk.ei && wk.j == wk.ej) {
+                if (++wk.r >= n_roots) {
+                    state = ST_NEXT;
+                    continue;
+                }
+                wk.level = root_level;
+                wk.i = T->root_i[wk.r];
+                wk.j = T->root_j[wk.r];
+                wk.elevel = root_level;
+                wk.ei = wk.i;
+                wk.ej = wk.j;
+                cell_next(wk.elevel, wk.ei, wk.ej, wk.si, wk.sj, root_level);
+                continue;
+            }
+            R w = ldexp((R)1, wk.level) / (R)p.pyr.n_leaf;
+            if (square_outside<R>(*T, ((R)wk.i + (R)0.5) * w, ((R)wk.j + (R)0.5) * w, (R)0.5 * w)) {
+                cell_next(wk.level, wk.i, wk.j, wk.si, wk.sj, root_level);
+                continue;
+            }
+            if (STATS)
+                ++st.node;
+            if (!node_test<R>(p, *T, wk, t_min, t_best)) {
+                cell_next(wk.level, wk.i, wk.j, wk.si, wk.sj, root_level);
+                continue;
+            }
+            if (wk.level > 0) {
+                // cell_down (displaced_intersector.cpp:92-97)
+                --wk.level;
+                wk.i = 2 * wk.i + wk.si;
+                wk.j = 2 * wk.j + wk.sj;
+                continue;
+            }
+            if (STATS)
+                ++st.leaf;
+            state = ST_LEAF;
+        }
+        if (__all(state == ST_DONE))
+            break;
+        if (state == ST_LEAF) {
+            Leaf<R> L;
+            leaf_setup<R>(p, *T, o, d, wk.i, wk.j, L);
+            R la = 0, lb = 0;
+            bool found;
+            if (p.leaf == 0)
+                found = search_leaf<R>(*T, L, t_min, t_best, la, lb, s_t, s_code, st);
+            else
+                found = leaf_two_triangle<R>(p, *T, L, o, d, t_min, t_best, la, lb);
+            if (found) {
+                hit_tri = k;
+                hit_i = wk.i;
+                hit_j = wk.j;
+                hit_a = la;
+                hit_b = lb;
+            }
+            if (found && p.any_hit) {
+                state = ST_DONE;
+            } else {
+                cell_next(wk.level, wk.i, wk.j, wk.si, wk.sj, root_level);
+                state = ST_WALK;
+            }
+        }
+    }
+
+    if (!valid)
+        return;
+    if (p.any_hit) {
+        p.occluded[rid] = hit_tri >= 0 ? 1 : 0;
+    } else {
+        const HitsDev &H = p.hits;
+        if (hit_tri < 0) {
+            H.t[rid] = INFINITY;
+            if (H.prim)
+                H.prim[rid] = -1;
+            if (H.s)
+                H.s[rid] = 0.f;
+            if (H.tt)
+                H.tt[rid] = 0.f;
+            if (H.bu)
+                H.bu[rid] = 0.f;
+            if (H.bv)
+                H.bv[rid] = 0.f;
+            if (H.nx)
+                H.nx[rid] = 0.f;
+            if (H.ny)
+                H.ny[rid] = 0.f;
+            if (H.nz)
+                H.nz[rid] = 0.f;
+            if (H.texel_i)
+                H.texel_i[rid] = 0;
+            if (H.texel_j)
+                H.texel_j[rid] = 0;
+        } else {
+            const TriRecord &TH = p.tris[hit_tri];
+            Leaf<R> L;
+            leaf_setup<R>(p, TH, o, d, hit_i, hit_j, L);
+            R s = L.slo + L.w * hit_a, t = L.tlo + L.w * hit_b;
+            // ng = unit (Su x Sv) at the root (displaced_intersector.cpp:363),
+            // from the texel-local partials; the two-triangle leaf's is its face normal.
+            R Sa[3], Sb[3];
+            R h = ((L.h00 + L.ha * hit_a) + L.hb * hit_b) + L.tw * hit_a * hit_b;
+            R hda = L.ha + L.tw * hit_b, hdb = L.hb + L.tw * hit_a;
+            if (p.leaf == 1) {
+                // face normal of the two-triangle split
+                bool upper = hit_a + hit_b > (R)1;
+                R hc[4] = {L.h00, L.h00 + L.ha, L.h00 + L.hb, L.h00 + L.ha + L.hb + L.tw};
+                R N[3], Na[3], Nb[3];
+                R Sc[4][3];
+                const R ca[4] = {0, 1, 0, 1}, cb[4] = {0, 0, 1, 1};
+                for (int c = 0; c < 4; ++c) {
+                    if (TH.const_normal) {
+                        for (int q = 0; q < 3; ++q)
+                            N[q] = L.N[q];
+                    } else {
+                        normal_at(L, ca[c], cb[c], N, Na, Nb);
+                    }
+                    for (int q = 0; q < 3; ++q)
+                        Sc[c][q] = ca[c] * L.E1[q] + cb[c] * L.E2[q] + hc[c] * N[q];
+                }
+                int A = upper ? 1 : 0, B = upper ? 3 : 1, C = 2;
+                for (int q = 0; q < 3; ++q) {
+                    Sa[q] = Sc[B][q] - Sc[A][q];
+                    Sb[q] = Sc[C][q] - Sc[A][q];
+                }
+            } else if (TH.const_normal) {
+                for (int q = 0; q < 3; ++q) {
+                    Sa[q] = L.E1[q] + hda * L.N[q];
+                    Sb[q] = L.E2[q] + hdb * L.N[q];
+                }
+            } else {
+                R N[3], Na[3], Nb[3];
+                normal_at(L, hit_a, hit_b, N, Na, Nb);
+                for (int q = 0; q < 3; ++q) {
+                    Sa[q] = L.E1[q] + hda * N[q] + h * Na[q];
+                    Sb[q] = L.E2[q] + hdb * N[q] + h * Nb[q];
+                }
+            }
+            R n[3] = {Sa[1] * Sb[2] - Sa[2] * Sb[1], Sa[2] * Sb[0] - Sa[0] * Sb[2], Sa[0] * Sb[1] - Sa[1] * Sb[0]};
+            R len = sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+            if (len > (R)0)
+                for (int q = 0; q < 3; ++q)
+                    n[q] /= len;
+            H.t[rid] = (float)t_best;
+            if (H.prim)
+                H.prim[rid] = hit_tri;
+            if (H.s)
+                H.s[rid] = (float)s;
+            if (H.tt)
+                H.tt[rid] = (float)t;
+            if (H.bu || H.bv) {
+                R ds = s - (R)TH.t0[0], dt = t - (R)TH.t0[1];
+                if (H.bu)
+                    H.bu[rid] = (float)((R)TH.J[0] * ds + (R)TH.J[1] * dt);
+                if (H.bv)
+                    H.bv[rid] = (float)((R)TH.J[2] * ds + (R)TH.J[3] * dt);
+            }
+            if (H.nx)
+                H.nx[rid] = (float)n[0];
+            if (H.ny)
+                H.ny[rid] = (float)n[1];
+            if (H.nz)
+                H.nz[rid] = (float)n[2];
+            if (H.texel_i)
+                H.texel_i[rid] = hit_i;
+            if (H.texel_j)
+                H.texel_j[rid] = hit_j;
+        }
+    }
+    if (STATS) {
+        const StatsDev &S = p.stats;
+        if (S.triangle_tests)
+            S.triangle_tests[rid] = st.tri;
+        if (S.node_tests)
+            S.node_tests[rid] = st.node;
+        if (S.leaf_tests)
+            S.leaf_tests[rid] = st.leaf;
+        if (S.sub_squares)
+            S.sub_squares[rid] = st.sub;
+        if (S.newton_runs)
+            S.newton_runs[rid] = st.runs;
+        if (S.newton_iters)
+            S.newton_iters[rid] = st.iters;
+    }
+}
+
+// ------------------------------------------------------- height range
+
+// classify_square in double (uv_clip.cpp:27-44): 0 outside, 1 inside, 2 straddle.
+__device__ inline int classify_d(const TriRecord &T, double cx, double cy, double s)
+{
+    if (cx + s <= T.dlo_d[0] || cx - s >= T.dhi_d[0] || cy + s <= T.dlo_d[1] || cy - s >= T.dhi_d[1])
+        return 0;
+    bool all_inside = true;
+    for (int k = 0; k < 3; ++k) {
+        double nx = T.dn_d[2 * k], ny = T.dn_d[2 * k + 1];
+        double proj = nx * cx + ny * cy;
+        double ext = (fabs(nx) + fabs(ny)) * s;
+        if (proj + ext <= T.dd_d[k])
+            return 0;
+        if (proj - ext < T.dd_d[k])
+            all_inside = false;
+    }
+    return all_inside ? 1 : 2;
+}
+
+// height_range_walk / triangle_height_range (node_bounds.cpp:218-249).
+__global__ void k_height_range(PyramidView P, const TriRecord *tris, int n_tri, float *h_range)
+{
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n_tri)
+        return;
+    const TriRecord &T = tris[k];
+    int stack_l[64], stack_i[64], stack_j[64];
+    int sp = 0;
+    for (int r = T.n_roots - 1; r >= 0; --r) {
+        stack_l[sp] = T.root_level;
+        stack_i[sp] = T.root_i[r];
+        stack_j[sp] = T.root_j[r];
+        ++sp;
+    }
+    float lo = INFINITY, hi = -INFINITY;
+    while (sp > 0) {
+        --sp;
+        int level = stack_l[sp], i = stack_i[sp], j = stack_j[sp];
+        double w = ldexp(1.0, level) / P.n_leaf;
+        int ov = classify_d(T, (i + 0.5) * w, (j + 0.5) * w, 0.5 * w);
+        if (ov == 0)
+            continue;
+        if (ov == 1 || level == 0) {
+            float2 mm = fetch_mm(P, level, i, j);
+            lo = fminf(lo, mm.x);
+            hi = fmaxf(hi, mm.y);
+            continue;
+        }
+        if (sp + 4 > 64)
+            continue; // cannot happen for levels below 20
+        for (int ch = 3; ch >= 0; --ch) {
+            stack_l[sp] = level - 1;
+            stack_i[sp] = 2 * i + (ch & 1);
+            stack_j[sp] = 2 * j + (ch >> 1);
+            ++sp;
+        }
+    }
+    h_range[2 * k] = lo;
+    h_range[2 * k + 1] = hi;
+}
+
+hipError_t launch_height_range(const PyramidView &P, const TriRecord *tris, int n_tri, float *h_range,
+                               hipStream_t stream)
+{
+    hipLaunchKernelGGL(k_height_range, dim3((n_tri + 63) / 64), dim3(64), 0, stream, P, tris, n_tri, h_range);
+    return hipGetLastError();
+}
+
+hipError_t launch_trace(const TraceParams &p, int precision, hipStream_t stream)
+{
+    if (p.n_rays <= 0)
+        return hipSuccess;
+    unsigned blocks = (unsigned)((p.n_rays + BLOCK - 1) / BLOCK);
+    bool stats = p.stats.node_tests || p.stats.leaf_tests || p.stats.sub_squares || p.stats.triangle_tests ||
+                 p.stats.newton_runs || p.stats.newton_iters;
+    if (precision == 1) {
+        if (stats)
+            hipLaunchKernelGGL((k_trace<double, true>), dim3(blocks), dim3(BLOCK), 0, stream, p);
+        else
+            hipLaunchKernelGGL((k_trace<double, false>), dim3(blocks), dim3(BLOCK), 0, stream, p);
+    } else {
+        if (stats)
+            hipLaunchKernelGGL((k_trace<float, true>), dim3(blocks), dim3(BLOCK), 0, stream, p);
+        else
+            hipLaunchKernelGGL((k_trace<float, false>), dim3(blocks), dim3(BLOCK), 0, stream, p);
+    }
+    return hipGetLastError();
+}
+
+} // namespace dmap_amd

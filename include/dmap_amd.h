@@ -1,0 +1,135 @@
+/*
+ * dmap_amd: ray / displaced-surface intersection on AMD Instinct MI355X
+ * (gfx950). C ABI of libdmap_amd.so.
+ *
+ * Each entry point replaces a reference interface (paths relative to the
+ * reference repository, code/cpp/src/):
+ *   dmap_tile_create      DisplacementAsset ctor + TaylorPyramid ctor
+ *                         (displacement_asset.cpp:16-35, taylor_pyramid.cpp:215-251,
+ *                         texture_grid.cpp:118-132)
+ *   dmap_object_create    ObjectIntersector ctor (displaced_intersector.cpp:466-479)
+ *                         over BaseTriangle (displaced_surface.cpp:54-91)
+ *   dmap_intersect_closest ObjectIntersector::intersect / intersect_flat
+ *                         (displaced_intersector.h:193-197, displaced_intersector.cpp:556-599)
+ *                         and, with prim_mask = k, DisplacedIntersector::intersect
+ *                         (displaced_intersector.h:137-138) on triangle k alone
+ *   dmap_intersect_any    the occluded callback (displaced_geometry.cpp:79-106)
+ *
+ * Conventions (as the reference's, displaced_intersector.h:107-114):
+ *   rays are in object space, d need not be unit length, t is in units of
+ *   d, hits satisfy t_min < t < t_max (open interval); a miss writes
+ *   t = +inf and prim = -1. st are tile coordinates, (bu, bv) the base
+ *   triangle's barycentric weights of p1 and p2, n the unit Su x Sv
+ *   (unoriented), (texel_i, texel_j) the leaf texel.
+ *
+ * Memory: ray and hit buffers are caller-owned DEVICE pointers (SoA, fp32),
+ * used asynchronously on the caller's HIP stream (NULL = the null stream).
+ * Tiles and objects own their device memory and stay resident until
+ * destroyed.
+ *
+ * Errors: every call returns DMAP_OK or an error code; dmap_last_error()
+ * gives the calling thread's last message. Nothing aborts the process.
+ * Threading: creation calls may run concurrently on different objects;
+ * queries on one object may run concurrently from several host threads on
+ * distinct streams (the reference's queries are const and reentrant).
+ */
+#ifndef DMAP_AMD_H
+#define DMAP_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DMAP_OK 0
+#define DMAP_ERR_INVALID 1   /* bad argument (the reference ASSERTs) */
+#define DMAP_ERR_HIP 2       /* a HIP runtime call failed */
+#define DMAP_ERR_MEMORY 3    /* device allocation failed */
+
+typedef struct dmap_ctx dmap_ctx;
+typedef struct dmap_tile dmap_tile;
+typedef struct dmap_object dmap_object;
+
+/* TraversalOptions (traversal_options.h:30-38). */
+typedef struct
+{
+    int bound;          /* 0 box, 1 slab, 2 both (default) */
+    int slab_max_level; /* the slab is read at levels <= this (default 1) */
+    int leaf;           /* 0 certified Newton (default), 1 two-triangle */
+    int precision;      /* 0 fp32 (default), 1 the same solve in fp64 on the device */
+} dmap_options;
+
+/* Device pointers, n entries each. */
+typedef struct
+{
+    const float *ox, *oy, *oz, *dx, *dy, *dz, *tmin, *tmax;
+} dmap_rays;
+
+/* Device pointers, n entries each; every field but t may be NULL. */
+typedef struct
+{
+    float *t, *s, *tt, *bu, *bv, *nx, *ny, *nz;
+    int32_t *prim, *texel_i, *texel_j;
+} dmap_hits;
+
+/* Per-ray work counters, TraversalStats (displaced_intersector.h:89-105);
+ * device pointers, any may be NULL. */
+typedef struct
+{
+    int32_t *triangle_tests, *node_tests, *leaf_tests, *sub_squares, *newton_runs, *newton_iters;
+} dmap_ray_stats;
+
+const char *dmap_last_error(void);
+const char *dmap_version(void);
+
+/* One HIP device. */
+int dmap_ctx_create(int device, dmap_ctx **out);
+void dmap_ctx_destroy(dmap_ctx *ctx);
+
+/* A resident displacement tile from n x n texels (n a power of two):
+ * h = strength * texel + offset (HeightGrid, displaced_surface.h:32-47),
+ * repeat 1 wraps, 0 clamps; build 0 fold, 1 direct (PyramidBuild,
+ * taylor_pyramid.h:41-45). texels is a host pointer, or a device pointer
+ * when texels_on_device is 1. Builds the node grid and the pyramid on the
+ * device, on `stream`, and waits for them. */
+int dmap_tile_create(dmap_ctx *ctx, const float *texels, int n, int texels_on_device, int repeat, double strength,
+                     double offset, int build, void *stream, dmap_tile **out);
+/* Re-runs the pyramid build from the resident node grid (for timing). */
+int dmap_tile_rebuild(dmap_tile *tile, void *stream);
+int dmap_tile_info(const dmap_tile *tile, int *n_leaf, int *n_levels, int64_t *device_bytes);
+/* The eight channels of one level (TaylorPyramid::levels, taylor_pyramid.h:51-61)
+ * to host memory: out holds 8 x m x m floats, channel-major, each channel
+ * row-major [j * m + i], channels h0 gu gv r ru rv h_min h_max. */
+int dmap_tile_read_level(const dmap_tile *tile, int level, float *out);
+/* The closed node grid ((n+1)^2 raw texel values) to host memory. */
+int dmap_tile_read_nodes(const dmap_tile *tile, float *out);
+void dmap_tile_destroy(dmap_tile *tile);
+
+/* An object: n_tri base triangles over one tile. positions, normals: 9
+ * doubles per triangle (p0 p1 p2, m0 m1 m2); uvs: 6 doubles per triangle
+ * (t0 t1 t2). Host pointers. */
+int dmap_object_create(dmap_ctx *ctx, dmap_tile *tile, int n_tri, const double *positions, const double *normals,
+                       const double *uvs, dmap_object **out);
+/* Per-triangle set-up: root level, root count, roots (4 x (i, j)), the
+ * object-space AABB (lo xyz, hi xyz, outward-rounded fp32) and the height
+ * range over the footprint. */
+int dmap_object_triangle_info(const dmap_object *obj, int k, int *root_level, int *n_roots, int32_t *roots,
+                              float *aabb, float *h_range);
+void dmap_object_destroy(dmap_object *obj);
+
+/* Closest hit of n rays. prim_mask -1 tests every triangle, k >= 0 only
+ * triangle k (the emitter-pdf door, displaced_area_light.cpp:174-186).
+ * options and stats may be NULL. */
+int dmap_intersect_closest(const dmap_object *obj, const dmap_rays *rays, int64_t n, const dmap_hits *hits,
+                           int prim_mask, const dmap_options *options, const dmap_ray_stats *stats, void *stream);
+
+/* Any hit: occluded[r] = 1 when the ray meets the surface in (tmin, tmax). */
+int dmap_intersect_any(const dmap_object *obj, const dmap_rays *rays, int64_t n, uint8_t *occluded, int prim_mask,
+                       const dmap_options *options, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* DMAP_AMD_H */
