@@ -1,0 +1,170 @@
+"""Closest-hit throughput on one MI355X for the incoherent displaced-surface
+workload (64 quads, 2048^2 fBm + ridged map, 4,194,304 rays: 2048 x 1024
+pinhole primaries and 2,097,152 cosine-weighted bounce rays).
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+
+A step is one closest-hit launch over all 4,194,304 rays, rays already
+resident in HBM. Prints one JSON line: Mrays/s, the traversal kernel's
+roofline position (algorithmic bytes from the oracle's work counters on a
+sample of the same rays), and the CPU oracle timed on the host's cores on
+that sample.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "closest-hit Mrays/s vs CPU ref; batched serving p50/p99 latency at fixed load"
+HBM_PEAK_GBS = 8000.0
+FP32_PEAK_TFLOPS = 157.3
+# Algorithmic bytes per ray: the ray in (8 fp32) and the hit out (8 fp32
+# + 3 int32 counted as 32 B), 32 B per fp32 node test (its 8-channel node),
+# 16 B per leaf test (its 4 corner heights).
+BYTES_RAY_IO = 32 + 32
+BYTES_NODE = 32
+BYTES_LEAF = 16
+# fp32 operations per unit of work, counted from the restated formulas
+# (node test with box + slab, sub-square bound, Newton iteration).
+FLOPS_NODE = 200
+FLOPS_SUB = 120
+FLOPS_NEWTON = 110
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-sample", type=int, default=4, help="oracle sample: every k-th ray")
+    ap.add_argument("--precision", default="fp32", choices=["fp32", "fp64"])
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus}); "
+              "rays shard by index range with no exchange, see DESIGN.md", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    import dmap_amd
+    from dmap_amd import workload as W
+
+    t0 = time.time()
+    wl = W.c2_incoherent()
+    n = wl.rays.shape[1]
+    log(f"[bench] workload {wl.description}: generated in {time.time() - t0:.1f}s")
+    tile = dmap_amd.DisplacementTile(wl.texels, wl.strength, wl.offset, wl.repeat, "fold")
+    obj = dmap_amd.ObjectIntersector(tile, wl.triangles)
+    rays = torch.from_numpy(wl.rays).cuda()
+    hits = obj.alloc_hits(n)
+    opt = dmap_amd.TraversalOptions(precision=args.precision)
+    stream = torch.cuda.current_stream()
+    torch.cuda.synchronize()
+
+    for _ in range(args.warmup):
+        obj.intersect(rays, options=opt, out=hits)
+    torch.cuda.synchronize()
+    log(f"[bench] warmup done ({args.warmup} steps)")
+
+    starts = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
+    ends = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    wall0 = time.perf_counter()
+    for k in range(args.steps):
+        starts[k].record(stream)
+        obj.intersect(rays, options=opt, out=hits)
+        ends[k].record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - wall0
+    launch_ms = [s.elapsed_time(e) for s, e in zip(starts, ends)]
+    avg_ms = float(np.mean(launch_ms))
+    ms_per_step = wall * 1e3 / args.steps
+    value = n / (ms_per_step * 1e-3) / 1e6
+    t_gpu = hits["f"][0].cpu().numpy()
+    log(f"[bench] {args.steps} steps: {ms_per_step:.3f} ms/step (kernel {avg_ms:.3f} ms), {value:.1f} Mrays/s, "
+        f"hit fraction {np.isfinite(t_gpu).mean():.4f}")
+
+    # Tile build (kernel #1) timing, for the record.
+    b0, b1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    tile.rebuild()
+    b0.record(stream)
+    for _ in range(5):
+        tile.rebuild()
+    b1.record(stream)
+    torch.cuda.synchronize()
+    build_ms = b0.elapsed_time(b1) / 5
+    nn = tile.n_leaf
+    build_bytes = 4 * (nn + 1) ** 2 + 32 * ((4 * nn * nn - 1) // 3)
+
+    cpu = None
+    roof = None
+    extra = {}
+    if not args.no_cpu:
+        import oracle
+        idx = np.arange(0, n, args.cpu_sample)
+        o = wl.rays[0:3, idx].T.astype(np.float64)
+        d = wl.rays[3:6, idx].T.astype(np.float64)
+        sc = oracle.Scene(wl.nodes, wl.strength, wl.offset, wl.repeat, wl.triangles)
+        threads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+        log(f"[bench] cpu oracle on {len(idx)} rays, {threads} threads")
+        c0 = time.perf_counter()
+        ref = sc.intersect(o, d, wl.rays[6, idx].astype(np.float64), wl.rays[7, idx].astype(np.float64),
+                           threads=threads, stats=True)
+        cpu_s = time.perf_counter() - c0
+        cpu_mrays = len(idx) / cpu_s / 1e6
+        st = ref["stats"]
+        node = float(st["node_tests"].mean())
+        leaf = float(st["leaf_tests"].mean())
+        sub = float(st["sub_squares"].mean())
+        its = float(st["newton_iters"].mean())
+        bytes_ray = BYTES_RAY_IO + BYTES_NODE * node + BYTES_LEAF * leaf
+        flops_ray = FLOPS_NODE * node + FLOPS_SUB * sub + FLOPS_NEWTON * its
+        achieved = bytes_ray * n / (avg_ms * 1e-3) / 1e9
+        roof = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+                "kernel": "k_trace<float>", "bytes_per_ray": round(bytes_ray, 1)}
+        extra["valu"] = {"achieved_tflops": round(flops_ray * n / (avg_ms * 1e-3) / 1e12, 3),
+                         "peak_tflops": FP32_PEAK_TFLOPS,
+                         "frac": round(flops_ray * n / (avg_ms * 1e-3) / 1e12 / FP32_PEAK_TFLOPS, 5),
+                         "flops_per_ray": round(flops_ray, 1)}
+        extra["work_per_ray"] = {"node_tests": round(node, 3), "leaf_tests": round(leaf, 4),
+                                 "sub_squares": round(sub, 3), "newton_iters": round(its, 3)}
+        # Parity of the timed result on the sample.
+        gt = t_gpu[idx].astype(np.float64)
+        both = np.isfinite(gt) & np.isfinite(ref["t"])
+        extra["sample_parity"] = {"hitmiss_flips": int((np.isfinite(gt) != np.isfinite(ref["t"])).sum()),
+                                  "worst_t_rel": float(np.max(np.abs(gt[both] - ref["t"][both]) / ref["t"][both]))
+                                  if both.any() else 0.0}
+        cpu = {"value": round(cpu_mrays, 4), "unit": "Mrays/s", "cores": threads, "kind": "port",
+               "sample": f"every {args.cpu_sample}th ray of the same 4,194,304 ({len(idx)} rays), "
+                         f"C++ double oracle, {threads} std::threads, {cpu_s:.2f}s",
+               "gpu_over_cpu": round(value / cpu_mrays, 1)}
+        log(f"[bench] cpu {cpu_mrays:.3f} Mrays/s; gpu/cpu {value / cpu_mrays:.1f}x")
+
+    out = {"metric": METRIC, "value": round(value, 2), "unit": "Mrays/s", "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "ms_per_step": round(ms_per_step, 4), "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": args.precision, "data": "synthetic",
+           "config": {"workload": "C2 incoherent closest-hit: 64 quads on [0,8]^2, 2048^2 fp32 fBm(6)+ridged map, "
+                                  "12 pyramid levels, 4194304 rays (2097152 pinhole primaries + 2097152 cosine "
+                                  "bounces), both node test, Newton leaf",
+                      "rays": n, "triangles": int(wl.triangles.shape[0]), "tile": int(tile.n_leaf)},
+           "roofline": roof, "cpu_baseline": cpu,
+           "pyramid_build": {"ms": round(build_ms, 4), "bytes": build_bytes,
+                             "GB_s": round(build_bytes / (build_ms * 1e-3) / 1e9, 1)},
+           **extra}
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,269 @@
+"""The oracle's walk and leaves (displaced_intersector.cpp, node_bounds.cpp
+restated) pinned by closed forms, brute force and the reference's own
+validation invariants (validate_traversal.cpp:148-461,
+validate_node_bounds.cpp:124-170, test_traversal_cost.cpp:242-297)."""
+import numpy as np
+import pytest
+
+import oracle
+from dmap_amd import workload as W
+
+N_TEX = 16
+
+
+def small_scene(n=N_TEX, amp=0.08, seed=5, repeat=True, flat=None):
+    rng = np.random.default_rng(seed)
+    tex = rng.random((n, n)).astype(np.float32) if flat is None else np.full((n, n), flat, np.float32)
+    nodes = W.close_tile(tex, repeat).astype(np.float64)
+    tris = W.quad_plane(1, 1.0)
+    return oracle.Scene(nodes, amp, 0.0, repeat, tris), nodes * amp, tris
+
+
+def ray_set(n, seed=9):
+    """Half from above the unit quad towards it, half near the surface in random directions
+    (the two distributions of traversal_test_util.cpp:83-111)."""
+    rng = np.random.default_rng(seed)
+    m = n // 2
+    o1 = np.column_stack([rng.uniform(-0.2, 1.2, m), rng.uniform(-0.2, 1.2, m), rng.uniform(0.5, 1.5, m)])
+    tgt = np.column_stack([rng.uniform(0, 1, m), rng.uniform(0, 1, m), np.zeros(m)])
+    d1 = tgt - o1
+    o2 = np.column_stack([rng.uniform(0, 1, n - m), rng.uniform(0, 1, n - m), rng.uniform(-0.1, 0.1, n - m)])
+    d2 = rng.normal(size=(n - m, 3))
+    d2 /= np.linalg.norm(d2, axis=1, keepdims=True)
+    return np.vstack([o1, o2]), np.vstack([d1, d2])
+
+
+def closed_form_hits(V, o, d):
+    """Nearest root of the ray against every texel's bilinear patch
+    S = (s, t, h(s, t)) (flat unit quad, normal +z): per texel the height
+    along the ray is quadratic in t. Returns t (inf on a miss)."""
+    n = V.shape[0] - 1
+    w = 1.0 / n
+    jj, ii = np.mgrid[0:n, 0:n]
+    ii, jj = ii.ravel(), jj.ravel()
+    h00, h10 = V[jj, ii], V[jj, ii + 1]
+    h01, h11 = V[jj + 1, ii], V[jj + 1, ii + 1]
+    ha, hb, T = h10 - h00, h01 - h00, h11 - h10 - h01 + h00
+    best = np.full(len(o), np.inf)
+    for r in range(len(o)):
+        a0, a1 = (o[r, 0] - ii * w) / w, d[r, 0] / w
+        b0, b1 = (o[r, 1] - jj * w) / w, d[r, 1] / w
+        c0 = h00 + ha * a0 + hb * b0 + T * a0 * b0
+        c1 = ha * a1 + hb * b1 + T * (a0 * b1 + a1 * b0)
+        c2 = T * a1 * b1
+        # (o_z - c0) + t (d_z - c1) - t^2 c2 = 0
+        A, B, C = -c2, d[r, 2] - c1, o[r, 2] - c0
+        ts = []
+        lin = np.abs(A) < 1e-300
+        with np.errstate(divide="ignore", invalid="ignore"):
+            t_lin = np.where(lin & (B != 0), -C / B, np.inf)
+            disc = B * B - 4 * A * C
+            sq = np.sqrt(np.maximum(disc, 0))
+            q = -0.5 * (B + np.sign(B + (B == 0)) * sq)
+            r1 = np.where(~lin & (disc >= 0), q / A, np.inf)
+            r2 = np.where(~lin & (disc >= 0) & (q != 0), C / q, np.inf)
+        for tt in (t_lin, r1, r2):
+            a, b = a0 + a1 * tt, b0 + b1 * tt
+            ok = (tt > 0) & np.isfinite(tt) & (a >= 0) & (a <= 1) & (b >= 0) & (b <= 1)
+            if ok.any():
+                best[r] = min(best[r], tt[ok].min())
+    return best
+
+
+def test_newton_leaf_matches_bilinear_closed_form():
+    sc, V, _ = small_scene()
+    o, d = ray_set(1200)
+    res = sc.intersect(o, d, threads=1)
+    ref = closed_form_hits(V, o, d)
+    hit_o, hit_r = np.isfinite(res["t"]), np.isfinite(ref)
+    dn = d / np.linalg.norm(d, axis=1, keepdims=True)
+    cos = np.abs(np.sum(dn * res["ng"], axis=1))
+    grazing = hit_o & (cos < 0.1)  # validate_traversal.cpp:116
+    agree = (hit_o == hit_r)
+    both = hit_o & hit_r
+    assert hit_r.sum() > 500
+    assert ((~agree) & ~grazing).sum() == 0
+    # Newton stops within residual_tolerance = 1e-10 x mean edge of the ray
+    # (displaced_intersector.cpp:150), a shift of tolerance / cos along it.
+    tol = 4 * 1.14e-10 / np.maximum(cos[both], 0.1) + 1e-12
+    dist = np.abs(res["t"][both] - ref[both]) * np.linalg.norm(d[both], axis=1)
+    assert (dist[~grazing[both]] <= tol[~grazing[both]]).all()
+
+
+def mt_mesh_hits(V, o, d):
+    """Moller-Trumbore over the texel mesh split along (i+1, j)-(i, j+1)
+    (displaced_tessellation.cpp:11-31): nearest t (inf on a miss)."""
+    n = V.shape[0] - 1
+    w = 1.0 / n
+    jj, ii = np.mgrid[0:n, 0:n]
+    ii, jj = ii.ravel(), jj.ravel()
+
+    def P(i, j):
+        return np.column_stack([i * w, j * w, V[j, i]])
+
+    S00, S10, S01, S11 = P(ii, jj), P(ii + 1, jj), P(ii, jj + 1), P(ii + 1, jj + 1)
+    A = np.vstack([S00, S10])
+    B = np.vstack([S10, S11])
+    C = np.vstack([S01, S01])
+    e1, e2 = B - A, C - A
+    best = np.full(len(o), np.inf)
+    for r in range(len(o)):
+        p = np.cross(d[r], e2)
+        det = np.sum(e1 * p, axis=1)
+        with np.errstate(divide="ignore", invalid="ignore"):
+            inv = 1.0 / det
+            s = o[r] - A
+            b1 = np.sum(s * p, axis=1) * inv
+            q = np.cross(s, e1)
+            b2 = (q @ d[r]) * inv
+            t = np.sum(e2 * q, axis=1) * inv
+        ok = (det != 0) & (b1 >= 0) & (b1 <= 1) & (b2 >= 0) & (b1 + b2 <= 1) & (t > 0)
+        if ok.any():
+            best[r] = t[ok].min()
+    return best
+
+
+def test_two_triangle_leaf_matches_mesh():
+    """validate_traversal.cpp:148-208: the two-triangle leaf is the texel mesh."""
+    sc, V, _ = small_scene()
+    o, d = ray_set(1200, seed=21)
+    res = sc.intersect(o, d, leaf="two_triangle", threads=1)
+    ref = mt_mesh_hits(V, o, d)
+    hit_o, hit_r = np.isfinite(res["t"]), np.isfinite(ref)
+    assert hit_r.sum() > 500
+    assert (hit_o == hit_r).sum() >= len(o) - 2  # edge-on cases only
+    both = hit_o & hit_r
+    assert np.allclose(res["t"][both], ref[both], rtol=1e-10, atol=1e-12)
+
+
+def test_never_miss_aimed_rays():
+    """validate_traversal.cpp:392-461: rays aimed at surface points hit at or before them."""
+    sc, _, tris = small_scene(seed=7)
+    rng = np.random.default_rng(3)
+    n = 3000
+    k = rng.integers(0, 2, n)
+    u = np.sqrt(rng.random(n))
+    b = np.column_stack([1 - u, u * rng.random(n)])
+    t = tris[k, 18:24].reshape(-1, 3, 2)
+    st = t[:, 0] + b[:, :1] * (t[:, 1] - t[:, 0]) + b[:, 1:2] * (t[:, 2] - t[:, 0])
+    S, _ = sc.surface_points(k.astype(np.int32), st)
+    dirs = rng.normal(size=(n, 3))
+    dirs /= np.linalg.norm(dirs, axis=1, keepdims=True)
+    ts = 1.14 * 10 ** (-1.5 + 2 * rng.random(n))
+    res = sc.intersect(S - ts[:, None] * dirs, dirs, threads=1)
+    cos = np.abs(np.sum(dirs * np.nan_to_num(res["ng"]), axis=1))
+    tol = 1e-9 * (ts + 1.5) + 4 * 1.14e-10 / np.maximum(cos, 1e-3)
+    assert np.isfinite(res["t"]).all()
+    assert (res["t"] <= ts + tol).all()
+
+
+@pytest.mark.parametrize("bound,k", [("box", 7), ("slab", 7), ("both", 0), ("both", 2), ("both", 7)])
+def test_bound_modes_give_identical_hits(bound, k):
+    """validate_traversal.cpp:320-390; and `both` never tests more nodes than `box`
+    (test_traversal_cost.cpp:268-297)."""
+    sc, _, _ = small_scene(n=64, seed=8)
+    o, d = ray_set(4000, seed=4)
+    ref = sc.intersect(o, d, bound="both", slab_max_level=1, stats=True)
+    got = sc.intersect(o, d, bound=bound, slab_max_level=k, stats=True)
+    assert (np.isfinite(ref["t"]) == np.isfinite(got["t"])).all()
+    both = np.isfinite(ref["t"])
+    assert np.allclose(got["t"][both], ref["t"][both], rtol=0, atol=1e-8)
+    if bound == "box":
+        assert (ref["stats"]["node_tests"] <= got["stats"]["node_tests"]).all()
+
+
+def test_flat_map_is_a_plane():
+    sc, _, _ = small_scene(flat=0.5, amp=0.1)
+    o, d = ray_set(2000, seed=13)
+    res = sc.intersect(o, d, threads=1)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        tp = (0.05 - o[:, 2]) / d[:, 2]
+    p = o + tp[:, None] * d
+    inside = (tp > 0) & (p[:, 0] >= 0) & (p[:, 0] <= 1) & (p[:, 1] >= 0) & (p[:, 1] <= 1)
+    edge = np.minimum.reduce([p[:, 0], 1 - p[:, 0], p[:, 1], 1 - p[:, 1], np.abs(p[:, 0] - p[:, 1])])
+    clear = np.abs(edge) > 1e-9
+    assert (np.isfinite(res["t"])[clear] == inside[clear]).all()
+    m = inside & clear
+    assert np.allclose(res["t"][m], tp[m], rtol=1e-9)
+    assert np.allclose(res["ng"][m], [0, 0, 1], atol=1e-12)
+
+
+def test_occluded_equals_closest_hit_exists():
+    sc, _, _ = small_scene(n=32)
+    o, d = ray_set(3000, seed=17)
+    res = sc.intersect(o, d)
+    occ = sc.occluded(o, d)
+    assert (occ == np.isfinite(res["t"])).all()
+    tmax = np.where(np.isfinite(res["t"]), res["t"] * 0.999, 1e9)
+    occ2 = sc.occluded(o, d, tmax=tmax)
+    # With t_max short of the first hit, only a hit nearer than that occludes: none.
+    assert not occ2[np.isfinite(res["t"])].any()
+
+
+def test_node_bounds_are_conservative():
+    """validate_node_bounds.cpp:124-170: dense surface points lie in every
+    node's box; rays through them have t inside box, slab and both."""
+    sc, V, tris = small_scene(n=16, seed=2)
+    rng = np.random.default_rng(1)
+    k = 0
+    checks = 0
+    for level in range(5):
+        m = 16 >> level
+        for (i, j) in rng.integers(0, m, size=(6, 2)):
+            nb = sc.node_bound(k, level, int(i), int(j))
+            w = 2.0 ** level / 16
+            g = (np.arange(9) + 0.5) / 9
+            ss, tt = np.meshgrid(i * w + g * w, j * w + g * w)
+            st = np.column_stack([ss.ravel(), tt.ravel()])
+            inside = st[:, 0] >= st[:, 1]  # triangle 0 of the quad: s >= t
+            st = st[inside]
+            if len(st) == 0:
+                continue
+            S, _ = sc.surface_points(np.zeros(len(st), np.int32), st)
+            # tangent coordinates of the flat unit quad are (x, y, z) itself
+            lo, hi = nb["box"]
+            assert (S >= lo - 1e-12).all() and (S <= hi + 1e-12).all()
+            for p in S[::7]:
+                dvec = rng.normal(size=3)
+                dvec /= np.linalg.norm(dvec)
+                origin = p - 0.7 * dvec
+                for mode in ("box", "slab", "both"):
+                    ok, t01 = sc.node_interval(k, level, int(i), int(j), origin, dvec, 0.0, np.inf, mode)
+                    assert ok and t01[0] - 1e-9 <= 0.7 <= t01[1] + 1e-9, (level, i, j, mode, t01)
+                    checks += 1
+    assert checks > 100
+
+
+def test_triangle_aabb_contains_surface():
+    sc, _, tris = small_scene(n=16, seed=4)
+    rng = np.random.default_rng(5)
+    for k in range(2):
+        info = sc.triangle_info(k)
+        u = np.sqrt(rng.random(2000))
+        b = np.column_stack([1 - u, u * rng.random(2000)])
+        t = tris[k, 18:24].reshape(3, 2)
+        st = t[0] + b[:, :1] * (t[1] - t[0]) + b[:, 1:2] * (t[2] - t[0])
+        S, _ = sc.surface_points(np.full(len(st), k, np.int32), st)
+        assert (S >= info["aabb"][0] - 1e-12).all() and (S <= info["aabb"][1] + 1e-12).all()
+        assert info["root_level"] == 3 and len(info["roots"]) == 4
+
+
+def test_classify_square_against_sampling():
+    rng = np.random.default_rng(6)
+    tri = np.array([0.1, 0.2, 0.8, 0.3, 0.3, 0.9])
+    for _ in range(300):
+        c = rng.random(2)
+        s = rng.uniform(0.01, 0.2)
+        cls = oracle.classify_square(tri, c[0], c[1], s)
+        g = np.linspace(-s, s, 41)
+        X, Y = np.meshgrid(c[0] + g, c[1] + g)
+        P = np.column_stack([X.ravel(), Y.ravel()])
+        a, b, cc = tri[0:2], tri[2:4], tri[4:6]
+
+        def side(p, q, r):
+            return (q[0] - p[0]) * (r[:, 1] - p[1]) - (q[1] - p[1]) * (r[:, 0] - p[0])
+        ins = (side(a, b, P) >= 0) & (side(b, cc, P) >= 0) & (side(cc, a, P) >= 0)
+        if cls == 0:
+            assert not ins.any()
+        elif cls == 1:
+            assert ins.all()
