@@ -422,13 +422,15 @@ __device__ bool node_test(const TraceParams &p, const TriRecord &T, const Walk<R
 // ------------------------------------------------------------------ leaf
 
 // Per-leaf constants of the certified Newton leaf, in texel-local
-// coordinates (a, b) in [0, 1]^2: S(a, b) - o = (P0 - o) + a E1 + b E2 +
-// h(a, b) N(a, b), E1 = w e1, E2 = w e2, P0 = P(i w, j w).
+// coordinates (a, b) in [0, 1]^2: S(a, b) - o = (A - o) + a E1 + b E2 +
+// (h(a, b) - shift) N(a, b), E1 = w e1, E2 = w e2, P0 = P(i w, j w), with the
+// anchor A = P0 + h00 N and shift = h00 under a constant normal, A = P0 and
+// shift = 0 otherwise. The anchor projections are formed in double.
 template <class R> struct Leaf
 {
     R c1, c2, cn;                     // d1, d2, dn . (P0 - o)
     R p1a, p1b, p2a, p2b, pna, pnb;   // d1, d2, dn . E1, E2
-    R h00, ha, hb, tw;                // h = h00 + ha a + hb b + tw a b
+    R h00, ha, hb, tw;                // h - shift = h00 + ha a + hb b + tw a b (shift: the anchor height)
     R n1, n2, nn;                     // d1, d2, dn . N (constant normal)
     R a1, a2, an;                     // sum |d_x| |N_x|
     R k1, k2, kn;                     // sum |d_x| N remainder_x
@@ -477,10 +479,19 @@ __device__ void leaf_setup(const TraceParams &p, const TriRecord &T, const doubl
     const PyramidView &P = p.pyr;
     double w = 1.0 / P.n_leaf;
     double slo = i * w, tlo = j * w;
+    // Corner heights h = scale * value + offset (HeightGrid::h at the corners).
+    double h00 = P.scale_d * (double)node_value(P, i, j) + P.offset_d;
+    double h10 = P.scale_d * (double)node_value(P, i + 1, j) + P.offset_d;
+    double h01 = P.scale_d * (double)node_value(P, i, j + 1) + P.offset_d;
+    double h11 = P.scale_d * (double)node_value(P, i + 1, j + 1) + P.offset_d;
+    // The anchor: the displaced texel corner P0 + h00 N under a constant
+    // normal (heights then enter relative to h00 and stay small), the base
+    // corner P0 otherwise.
+    double shift = T.const_normal ? h00 : 0.0;
     double P0[3], rel[3];
     for (int q = 0; q < 3; ++q) {
         P0[q] = T.q0[q] + slo * T.e1[q] + tlo * T.e2[q];
-        rel[q] = P0[q] - o[q];
+        rel[q] = P0[q] + (T.const_normal ? h00 * T.nhat_d[q] : 0.0) - o[q];
     }
     double d_len = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
     double dn[3] = {d[0] / d_len, d[1] / d_len, d[2] / d_len}, d1[3], d2[3];
@@ -503,15 +514,12 @@ __device__ void leaf_setup(const TraceParams &p, const TriRecord &T, const doubl
         L.d2[q] = (R)d2[q];
         L.dn[q] = (R)dn[q];
     }
-    // Corner heights h = scale * value + offset (HeightGrid::h at the corners).
-    R sc = sizeof(R) == 8 ? (R)P.scale_d : (R)P.scale, of = sizeof(R) == 8 ? (R)P.offset_d : (R)P.offset;
-    R h00 = sc * (R)node_value(P, i, j) + of, h10 = sc * (R)node_value(P, i + 1, j) + of;
-    R h01 = sc * (R)node_value(P, i, j + 1) + of, h11 = sc * (R)node_value(P, i + 1, j + 1) + of;
-    L.h00 = h00;
-    L.ha = h10 - h00;
-    L.hb = h01 - h00;
-    L.tw = ((h11 - h10) - h01) + h00;
-    L.hmax = fmax(fmax(fabs(h00), fabs(h10)), fmax(fabs(h01), fabs(h11)));
+    // h - shift = h00 + ha a + hb b + tw a b over the texel.
+    L.h00 = (R)(h00 - shift);
+    L.ha = (R)(h10 - h00);
+    L.hb = (R)(h01 - h00);
+    L.tw = (R)(((h11 - h10) - h01) + h00);
+    L.hmax = (R)fmax(fmax(fabs(h00 - shift), fabs(h10 - shift)), fmax(fabs(h01 - shift), fabs(h11 - shift)));
     if (T.const_normal) {
         double N[3] = {T.nhat_d[0], T.nhat_d[1], T.nhat_d[2]};
         double nk = T.nhat_kd;
@@ -647,6 +655,62 @@ __device__ bool sub_square_bound(const TriRecord &T, const Leaf<R> &L, R a, R b,
     return t_lo < t_max && t_hi > t_min;
 }
 
+// F = ((S - o) . d1, (S - o) . d2), the distance along the ray and the
+// Jacobian of F at texel-local (a, b) (displaced_intersector.cpp:337-375).
+template <class R>
+__device__ inline void newton_eval(const TriRecord &T, const Leaf<R> &L, R a, R b, R &F1, R &F2, R &along, R &J00,
+                                   R &J01, R &J10, R &J11)
+{
+    R h = ((L.h00 + L.ha * a) + L.hb * b) + L.tw * a * b;
+    R hda = L.ha + L.tw * b, hdb = L.hb + L.tw * a;
+    if (T.const_normal) {
+        F1 = L.c1 + a * L.p1a + b * L.p1b + h * L.n1;
+        F2 = L.c2 + a * L.p2a + b * L.p2b + h * L.n2;
+        along = L.cn + a * L.pna + b * L.pnb + h * L.nn;
+        J00 = L.p1a + hda * L.n1;
+        J01 = L.p1b + hdb * L.n1;
+        J10 = L.p2a + hda * L.n2;
+        J11 = L.p2b + hdb * L.n2;
+    } else {
+        R N[3], Na[3], Nb[3];
+        normal_at(L, a, b, N, Na, Nb);
+        R Sl[3], Sa[3], Sb[3];
+        for (int q = 0; q < 3; ++q) {
+            Sl[q] = a * L.E1[q] + b * L.E2[q] + h * N[q];
+            Sa[q] = L.E1[q] + hda * N[q] + h * Na[q];
+            Sb[q] = L.E2[q] + hdb * N[q] + h * Nb[q];
+        }
+        auto dt = [](const R *x, const R *y) { return x[0] * y[0] + x[1] * y[1] + x[2] * y[2]; };
+        F1 = L.c1 + dt(L.d1, Sl);
+        F2 = L.c2 + dt(L.d2, Sl);
+        along = L.cn + dt(L.dn, Sl);
+        J00 = dt(L.d1, Sa);
+        J01 = dt(L.d1, Sb);
+        J10 = dt(L.d2, Sa);
+        J11 = dt(L.d2, Sb);
+    }
+}
+
+// Newton steps in double from an accepted root, for the reported values
+// only: the fp32 iteration stops at a residual of a few ulps of its
+// magnitudes, two double steps bring the root to the double solve's. A
+// step that would leave the texel is not taken.
+__device__ inline void polish_root(const TriRecord &T, const Leaf<double> &L, double &a, double &b)
+{
+    for (int it = 0; it < 2; ++it) {
+        double F1, F2, along, J00, J01, J10, J11;
+        newton_eval<double>(T, L, a, b, F1, F2, along, J00, J01, J10, J11);
+        double det = J00 * J11 - J01 * J10;
+        if (det == 0.0)
+            return;
+        double na = a - (J11 * F1 - J01 * F2) / det, nb = b - (-J10 * F1 + J00 * F2) / det;
+        if (!(na >= 0.0 && na <= 1.0 && nb >= 0.0 && nb <= 1.0))
+            return;
+        a = na;
+        b = nb;
+    }
+}
+
 // newton_from (displaced_intersector.cpp:325-388) in texel-local
 // coordinates. On success writes the root (a, b) and its t.
 template <class R>
@@ -657,35 +721,8 @@ __device__ bool newton_from(const TriRecord &T, const Leaf<R> &L, R a, R b, R t_
     int err_streak = 0, behind_streak = 0, invalid_streak = 0;
     for (int iter = 0; iter < MAX_ITERS; ++iter) {
         ++iters;
-        R h = ((L.h00 + L.ha * a) + L.hb * b) + L.tw * a * b;
-        R hda = L.ha + L.tw * b, hdb = L.hb + L.tw * a;
         R F1, F2, along, J00, J01, J10, J11;
-        if (T.const_normal) {
-            F1 = L.c1 + a * L.p1a + b * L.p1b + h * L.n1;
-            F2 = L.c2 + a * L.p2a + b * L.p2b + h * L.n2;
-            along = L.cn + a * L.pna + b * L.pnb + h * L.nn;
-            J00 = L.p1a + hda * L.n1;
-            J01 = L.p1b + hdb * L.n1;
-            J10 = L.p2a + hda * L.n2;
-            J11 = L.p2b + hdb * L.n2;
-        } else {
-            R N[3], Na[3], Nb[3];
-            normal_at(L, a, b, N, Na, Nb);
-            R Sl[3], Sa[3], Sb[3];
-            for (int q = 0; q < 3; ++q) {
-                Sl[q] = a * L.E1[q] + b * L.E2[q] + h * N[q];
-                Sa[q] = L.E1[q] + hda * N[q] + h * Na[q];
-                Sb[q] = L.E2[q] + hdb * N[q] + h * Nb[q];
-            }
-            auto dt = [](const R *x, const R *y) { return x[0] * y[0] + x[1] * y[1] + x[2] * y[2]; };
-            F1 = L.c1 + dt(L.d1, Sl);
-            F2 = L.c2 + dt(L.d2, Sl);
-            along = L.cn + dt(L.dn, Sl);
-            J00 = dt(L.d1, Sa);
-            J01 = dt(L.d1, Sb);
-            J10 = dt(L.d2, Sa);
-            J11 = dt(L.d2, Sb);
-        }
+        newton_eval<R>(T, L, a, b, F1, F2, along, J00, J01, J10, J11);
         R err2 = F1 * F1 + F2 * F2;
         err_streak = err2 > prev_err2 ? err_streak + 1 : 0;
         behind_streak = along < (R)0 ? behind_streak + 1 : 0;
@@ -1068,22 +1105,26 @@ __global__ void __launch_bounds__(BLOCK) k_trace(TraceParams p)
             if (H.texel_j)
                 H.texel_j[rid] = 0;
         } else {
+            // The reported hit, in double: the root polished (Newton leaf) or the
+            // face crossing recomputed (two-triangle leaf), then st, the chart's
+            // barycentrics and ng = unit (Su x Sv) at the root
+            // (displaced_intersector.cpp:357-366, 413-421).
             const TriRecord &TH = p.tris[hit_tri];
-            Leaf<R> L;
-            leaf_setup<R>(p, TH, o, d, hit_i, hit_j, L);
-            R s = L.slo + L.w * hit_a, t = L.tlo + L.w * hit_b;
-            // ng = unit (Su x Sv) at the root (displaced_intersector.cpp:363),
-            // from the texel-local partials; the two-triangle leaf's is its face normal.
-            R Sa[3], Sb[3];
-            R h = ((L.h00 + L.ha * hit_a) + L.hb * hit_b) + L.tw * hit_a * hit_b;
-            R hda = L.ha + L.tw * hit_b, hdb = L.hb + L.tw * hit_a;
+            Leaf<double> L;
+            leaf_setup<double>(p, TH, o, d, hit_i, hit_j, L);
+            double a = (double)hit_a, b = (double)hit_b, t_hit = (double)t_best;
+            double Sa[3], Sb[3];
             if (p.leaf == 1) {
-                // face normal of the two-triangle split
-                bool upper = hit_a + hit_b > (R)1;
-                R hc[4] = {L.h00, L.h00 + L.ha, L.h00 + L.hb, L.h00 + L.ha + L.hb + L.tw};
-                R N[3], Na[3], Nb[3];
-                R Sc[4][3];
-                const R ca[4] = {0, 1, 0, 1}, cb[4] = {0, 0, 1, 1};
+                double ra, rb, tm = (double)INFINITY;
+                if (leaf_two_triangle<double>(p, TH, L, o, d, (double)t_min, tm, ra, rb)) {
+                    a = ra;
+                    b = rb;
+                    t_hit = tm;
+                }
+                bool upper = a + b > 1.0;
+                double hc[4] = {L.h00, L.h00 + L.ha, L.h00 + L.hb, L.h00 + L.ha + L.hb + L.tw};
+                double N[3], Na[3], Nb[3], Sc[4][3];
+                const double ca[4] = {0, 1, 0, 1}, cb[4] = {0, 0, 1, 1};
                 for (int c = 0; c < 4; ++c) {
                     if (TH.const_normal) {
                         for (int q = 0; q < 3; ++q)
@@ -1099,25 +1140,35 @@ __global__ void __launch_bounds__(BLOCK) k_trace(TraceParams p)
                     Sa[q] = Sc[B][q] - Sc[A][q];
                     Sb[q] = Sc[C][q] - Sc[A][q];
                 }
-            } else if (TH.const_normal) {
-                for (int q = 0; q < 3; ++q) {
-                    Sa[q] = L.E1[q] + hda * L.N[q];
-                    Sb[q] = L.E2[q] + hdb * L.N[q];
-                }
             } else {
-                R N[3], Na[3], Nb[3];
-                normal_at(L, hit_a, hit_b, N, Na, Nb);
-                for (int q = 0; q < 3; ++q) {
-                    Sa[q] = L.E1[q] + hda * N[q] + h * Na[q];
-                    Sb[q] = L.E2[q] + hdb * N[q] + h * Nb[q];
+                polish_root(TH, L, a, b);
+                double F1, F2, along, J00, J01, J10, J11;
+                newton_eval<double>(TH, L, a, b, F1, F2, along, J00, J01, J10, J11);
+                t_hit = along / L.d_len;
+                double h = ((L.h00 + L.ha * a) + L.hb * b) + L.tw * a * b;
+                double hda = L.ha + L.tw * b, hdb = L.hb + L.tw * a;
+                if (TH.const_normal) {
+                    for (int q = 0; q < 3; ++q) {
+                        Sa[q] = L.E1[q] + hda * L.N[q];
+                        Sb[q] = L.E2[q] + hdb * L.N[q];
+                    }
+                } else {
+                    double N[3], Na[3], Nb[3];
+                    normal_at(L, a, b, N, Na, Nb);
+                    for (int q = 0; q < 3; ++q) {
+                        Sa[q] = L.E1[q] + hda * N[q] + h * Na[q];
+                        Sb[q] = L.E2[q] + hdb * N[q] + h * Nb[q];
+                    }
                 }
             }
-            R n[3] = {Sa[1] * Sb[2] - Sa[2] * Sb[1], Sa[2] * Sb[0] - Sa[0] * Sb[2], Sa[0] * Sb[1] - Sa[1] * Sb[0]};
-            R len = sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
-            if (len > (R)0)
+            double s = L.slo + L.w * a, t = L.tlo + L.w * b;
+            double n[3] = {Sa[1] * Sb[2] - Sa[2] * Sb[1], Sa[2] * Sb[0] - Sa[0] * Sb[2],
+                           Sa[0] * Sb[1] - Sa[1] * Sb[0]};
+            double len = sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+            if (len > 0.0)
                 for (int q = 0; q < 3; ++q)
                     n[q] /= len;
-            H.t[rid] = (float)t_best;
+            H.t[rid] = (float)t_hit;
             if (H.prim)
                 H.prim[rid] = hit_tri;
             if (H.s)
@@ -1125,11 +1176,11 @@ __global__ void __launch_bounds__(BLOCK) k_trace(TraceParams p)
             if (H.tt)
                 H.tt[rid] = (float)t;
             if (H.bu || H.bv) {
-                R ds = s - (R)TH.t0[0], dt = t - (R)TH.t0[1];
+                double ds = s - TH.t0[0], dt = t - TH.t0[1];
                 if (H.bu)
-                    H.bu[rid] = (float)((R)TH.J[0] * ds + (R)TH.J[1] * dt);
+                    H.bu[rid] = (float)(TH.J[0] * ds + TH.J[1] * dt);
                 if (H.bv)
-                    H.bv[rid] = (float)((R)TH.J[2] * ds + (R)TH.J[3] * dt);
+                    H.bv[rid] = (float)(TH.J[2] * ds + TH.J[3] * dt);
             }
             if (H.nx)
                 H.nx[rid] = (float)n[0];

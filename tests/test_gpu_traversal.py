@@ -38,7 +38,7 @@ def report(cmp):
 def test_c1_closest_hit_parity(torch_gpu):
     wl = W.c1_small()
     g, ref, d = run_both(wl)
-    cmp = compare(g, ref, d, wl.triangles)
+    cmp = compare(g, ref, d, wl.triangles, wl.texels.shape[0])
     print(report(cmp))
     assert cmp["hits_ref"] > 40000
     assert cmp["unexplained"] == 0, report(cmp)
@@ -50,6 +50,6 @@ def test_c2_small_closest_hit_parity(torch_gpu):
     """The incoherent workload's structure at 256^2 and 2 x 64k rays."""
     wl = W.c2_incoherent(n=256, width=256, height=256, n_bounce=65536)
     g, ref, d = run_both(wl)
-    cmp = compare(g, ref, d, wl.triangles)
+    cmp = compare(g, ref, d, wl.triangles, wl.texels.shape[0])
     print(report(cmp))
     assert cmp["unexplained"] == 0, report(cmp)
