@@ -45,10 +45,13 @@ struct dmap_object
     dmap_ctx *ctx = nullptr;
     dmap_tile *tile = nullptr;
     int n_tri = 0;
+    int const_normal = 0;
     std::vector<TriRecord> host;
     std::vector<float> boxes_host, h_range_host;
+    std::vector<BvhNode> bvh_host;
     TriRecord *tris = nullptr;
     float *boxes = nullptr;
+    BvhNode *bvh = nullptr;
 };
 
 namespace
@@ -279,6 +282,9 @@ bool fill_record(const HostTri &h, int n_leaf, TriRecord &R)
         return false;
     if (std::abs(i_lo) > (1 << 28) || std::abs(j_lo) > (1 << 28))
         return false;
+    // The walk keeps four bits per level below the root in 64 bits.
+    if (level > 16)
+        return false;
     R.root_level = level;
     R.n_roots = 0;
     for (int64_t j = j_lo; j <= j_hi; ++j)
@@ -367,6 +373,55 @@ void triangle_box(const HostTri &tri, double h_min, double h_max, float out[6])
 }
 
 bool is_pow2(int n) { return n >= 1 && (n & (n - 1)) == 0; }
+
+// Top-level BVH over the triangles' boxes: median split of the box centres
+// along the longest axis of their spread, one triangle per leaf. It stands
+// in for the reference's embree scene over the same boxes
+// (ObjectIntersector::build_bvh, displaced_intersector.cpp:540-554).
+int build_bvh_node(std::vector<BvhNode> &nodes, const std::vector<float> &boxes, std::vector<int> &idx, int lo, int hi,
+                   int parent)
+{
+    int me = (int)nodes.size();
+    nodes.push_back(BvhNode{});
+    BvhNode nd;
+    nd.parent = parent;
+    float blo[3] = {INFINITY, INFINITY, INFINITY}, bhi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    float clo[3] = {INFINITY, INFINITY, INFINITY}, chi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (int q = lo; q < hi; ++q) {
+        const float *b = &boxes[6 * idx[q]];
+        for (int a = 0; a < 3; ++a) {
+            blo[a] = std::min(blo[a], b[a]);
+            bhi[a] = std::max(bhi[a], b[3 + a]);
+            float c = 0.5f * (b[a] + b[3 + a]);
+            clo[a] = std::min(clo[a], c);
+            chi[a] = std::max(chi[a], c);
+        }
+    }
+    for (int a = 0; a < 3; ++a) {
+        nd.lo[a] = blo[a];
+        nd.hi[a] = bhi[a];
+    }
+    if (hi - lo == 1) {
+        nd.left = ~idx[lo];
+        nd.right = -1;
+        nd.axis = 0;
+        nodes[me] = nd;
+        return me;
+    }
+    int axis = 0;
+    for (int a = 1; a < 3; ++a)
+        if (chi[a] - clo[a] > chi[axis] - clo[axis])
+            axis = a;
+    int mid = (lo + hi) / 2;
+    std::nth_element(idx.begin() + lo, idx.begin() + mid, idx.begin() + hi, [&](int x, int y) {
+        return boxes[6 * x + axis] + boxes[6 * x + 3 + axis] < boxes[6 * y + axis] + boxes[6 * y + 3 + axis];
+    });
+    nd.axis = axis;
+    nd.left = build_bvh_node(nodes, boxes, idx, lo, mid, me);
+    nd.right = build_bvh_node(nodes, boxes, idx, mid, hi, me);
+    nodes[me] = nd;
+    return me;
+}
 
 } // namespace
 
@@ -568,6 +623,7 @@ static void object_free(dmap_object *o)
         return;
     hipFree(o->tris);
     hipFree(o->boxes);
+    hipFree(o->bvh);
     delete o;
 }
 
@@ -628,7 +684,19 @@ int dmap_object_create(dmap_ctx *ctx, dmap_tile *tile, int n_tri, const double *
             o->host[k].aabb_hi[q] = o->boxes_host[6 * k + 3 + q];
         }
     }
-    e = hipMemcpy(o->boxes, o->boxes_host.data(), sizeof(float) * 6 * n_tri, hipMemcpyHostToDevice);
+    std::vector<int> idx(n_tri);
+    for (int k = 0; k < n_tri; ++k)
+        idx[k] = k;
+    o->bvh_host.reserve(2 * (size_t)n_tri);
+    build_bvh_node(o->bvh_host, o->boxes_host, idx, 0, n_tri, -1);
+    o->const_normal = 1;
+    for (int k = 0; k < n_tri; ++k)
+        o->const_normal &= o->host[k].const_normal;
+    e = hipMalloc(&o->bvh, sizeof(BvhNode) * o->bvh_host.size());
+    if (e == hipSuccess)
+        e = hipMemcpy(o->bvh, o->bvh_host.data(), sizeof(BvhNode) * o->bvh_host.size(), hipMemcpyHostToDevice);
+    if (e == hipSuccess)
+        e = hipMemcpy(o->boxes, o->boxes_host.data(), sizeof(float) * 6 * n_tri, hipMemcpyHostToDevice);
     if (e == hipSuccess)
         e = hipMemcpy(o->tris, o->host.data(), sizeof(TriRecord) * n_tri, hipMemcpyHostToDevice);
     if (e != hipSuccess) {
@@ -704,7 +772,9 @@ static int trace(const dmap_object *obj, const dmap_rays *rays, int64_t n, const
     p.pyr = view_of(obj->tile);
     p.tris = obj->tris;
     p.boxes = obj->boxes;
+    p.bvh = obj->bvh;
     p.n_tri = obj->n_tri;
+    p.const_normal = obj->const_normal;
     p.prim_mask = prim_mask;
     p.bound = o.bound;
     p.slab_max_level = o.slab_max_level;
@@ -726,8 +796,8 @@ static int trace(const dmap_object *obj, const dmap_rays *rays, int64_t n, const
 int dmap_intersect_closest(const dmap_object *obj, const dmap_rays *rays, int64_t n, const dmap_hits *hits,
                            int prim_mask, const dmap_options *options, const dmap_ray_stats *stats, void *stream)
 {
-    if (!hits || !hits->t)
-        return fail(DMAP_ERR_INVALID, "dmap_intersect_closest: hits->t is NULL");
+    if (!hits || !hits->t || !hits->s || !hits->tt || !hits->prim || !hits->texel_i || !hits->texel_j)
+        return fail(DMAP_ERR_INVALID, "dmap_intersect_closest: hits t, s, tt, prim, texel_i and texel_j are required");
     return trace(obj, rays, n, hits, nullptr, prim_mask, options, stats, stream);
 }
 

@@ -44,12 +44,23 @@ struct StatsDev
     int32_t *triangle_tests, *node_tests, *leaf_tests, *sub_squares, *newton_runs, *newton_iters;
 };
 
+// A node of the object's top-level BVH over the triangles' boxes:
+// internal nodes have two children, a leaf holds one triangle
+// (left = ~triangle, right = -1). Node 0 is the root.
+struct BvhNode
+{
+    float lo[3], hi[3];
+    int parent, left, right, axis;
+};
+
 struct TraceParams
 {
     PyramidView pyr;
     const TriRecord *tris;
     const float *boxes; // n_tri x 6: lo xyz, hi xyz
+    const BvhNode *bvh;
     int n_tri;
+    int const_normal; // every triangle has a constant interpolated normal
     int prim_mask; // -1: every triangle; k: triangle k only
     int bound;     // 0 box, 1 slab, 2 both
     int slab_max_level;
@@ -63,6 +74,8 @@ struct TraceParams
 };
 
 // Closest hit / any hit over the object; precision 0 = fp32, 1 = fp64.
+// A closest-hit launch is followed by the finalize kernel, which turns the
+// raw record (t, prim, texel, texel-local root in s/tt) into the hit.
 hipError_t launch_trace(const TraceParams &p, int precision, hipStream_t stream);
 
 } // namespace dmap_amd

@@ -193,40 +193,13 @@ template <class R> __device__ void normal_mform(const TriRecord &T, R cx, R cy, 
     }
 }
 
-// Per-walk state of one lane.
+// The ray of one lane in a triangle's tangent space.
 template <class R> struct Walk
 {
-    R o[3], d[3], inv[3]; // the ray in tangent space
+    R o[3], d[3], inv[3];
     R omag, dmag;
-    int si, sj;
-    int level, i, j;    // current cell
-    int elevel, ei, ej; // the cell that ends the current root's subtree
-    int r;              // current root
+    int si, sj; // the ray runs towards -i (-j): the far child along i (j) is met first
 };
-
-__device__ inline void cell_next(int &level, int &i, int &j, int si, int sj, int root_level)
-{
-    // cell_next (displaced_intersector.cpp:99-121); floor_div2 is the
-    // arithmetic shift, floor_mod2 the low bit.
-    while (true) {
-        int code = 2 * ((i + si) & 1) + ((j + sj) & 1);
-        if (code == 1) {
-            j += sj ? 1 : -1;
-            i += si ? -1 : 1;
-            return;
-        }
-        if (code == 3) {
-            ++level;
-            i >>= 1;
-            j >>= 1;
-            if (level > root_level)
-                return;
-            continue;
-        }
-        j += sj ? -1 : 1;
-        return;
-    }
-}
 
 // classify_square (uv_clip.cpp:27-44), Outside only, with rounding room so
 // that a cell the double test keeps is kept.
@@ -290,16 +263,17 @@ __device__ inline float4 fetch_tp(const PyramidView &P, int level, int i, int j)
     return P.tp[idx];
 }
 
-// node_bound + node_interval (node_bounds.cpp:83-109, 111-211) for the
-// current cell of a walk. Returns whether the ray can meet the surface
-// over the cell within [t_min, t_max].
-template <class R>
-__device__ bool node_test(const TraceParams &p, const TriRecord &T, const Walk<R> &wk, R t_min, R t_max)
+// node_bound + node_interval (node_bounds.cpp:83-109, 111-211) for cell
+// (level, ci, cj) with its min-max pair mm and, when the slab is read, its
+// Taylor quad tp. Returns whether the ray can meet the surface over the
+// cell within [t_min, t_max].
+template <class R, bool CN>
+__device__ bool cell_test(const TraceParams &p, const TriRecord &T, const Walk<R> &wk, int level, int ci, int cj,
+                          float2 mm, float4 tp, R t_min, R t_max)
 {
     const PyramidView &P = p.pyr;
-    int level = wk.level;
     R w = ldexp((R)1, level) / (R)P.n_leaf;
-    R clo_x = (R)wk.i * w, clo_y = (R)wk.j * w;
+    R clo_x = (R)ci * w, clo_y = (R)cj * w;
     // The cell clipped to the triangle's texture bounding box (node_bounds.cpp:94-99).
     R rlo_x = fmax(clo_x, (R)T.dlo[0]), rlo_y = fmax(clo_y, (R)T.dlo[1]);
     R rhi_x = fmin(clo_x + w, (R)T.dhi[0]), rhi_y = fmin(clo_y + w, (R)T.dhi[1]);
@@ -313,7 +287,7 @@ __device__ bool node_test(const TraceParams &p, const TriRecord &T, const Walk<R
     R hx = fmax((R)0.5 * (rhi_x - rlo_x), (R)0), hy = fmax((R)0.5 * (rhi_y - rlo_y), (R)0);
 
     MForm<R> m;
-    if (T.const_normal) {
+    if (CN || T.const_normal) {
         for (int q = 0; q < 3; ++q) {
             m.c[q] = sizeof(R) == 8 ? (R)T.mc_d[q] : (R)T.mc[q];
             m.a[q] = 0;
@@ -324,7 +298,6 @@ __device__ bool node_test(const TraceParams &p, const TriRecord &T, const Walk<R
         normal_mform<R>(T, cx, cy, hx, hy, m);
     }
 
-    float2 mm = fetch_mm(P, level, wk.i, wk.j);
     R hc = (R)0.5 * ((R)mm.x + (R)mm.y), hr = (R)0.5 * ((R)mm.y - (R)mm.x);
     // box = hull(p + h m): p = (cx +- hx, cy +- hy, 0), h = hc +- hr.
     R blo[3], bhi[3];
@@ -364,13 +337,12 @@ __device__ bool node_test(const TraceParams &p, const TriRecord &T, const Walk<R
     for (int q = 0; q < 3; ++q)
         anchor[q] = wk.o[q] + t0 * wk.d[q];
     R span = t1 - t0;
-    float4 tp = fetch_tp(P, level, wk.i, wk.j);
     R h0 = tp.x, gu = tp.y, gv = tp.z, rr = tp.w;
     if (level >= P.n_levels) {
         gu = 0;
         gv = 0;
     }
-    R ucx = ((R)wk.i + (R)0.5) * w, ucy = ((R)wk.j + (R)0.5) * w; // nb.centre
+    R ucx = ((R)ci + (R)0.5) * w, ucy = ((R)cj + (R)0.5) * w; // nb.centre
     IvR<R> mx{m.c[0] - (fabs(m.a[0]) + fabs(m.b[0]) + m.k[0]), m.c[0] + (fabs(m.a[0]) + fabs(m.b[0]) + m.k[0])};
     IvR<R> my{m.c[1] - (fabs(m.a[1]) + fabs(m.b[1]) + m.k[1]), m.c[1] + (fabs(m.a[1]) + fabs(m.b[1]) + m.k[1])};
     IvR<R> mz{m.c[2] - (fabs(m.a[2]) + fabs(m.b[2]) + m.k[2]), m.c[2] + (fabs(m.a[2]) + fabs(m.b[2]) + m.k[2])};
@@ -595,7 +567,7 @@ template <class R> __device__ void normal_texel_form(const Leaf<R> &L, MForm<R> 
 
 // sub_square_bound (displaced_intersector.cpp:266-290) on the sub-square
 // with centre (a, b) and half-width rho (texel units).
-template <class R>
+template <class R, bool CN>
 __device__ bool sub_square_bound(const TriRecord &T, const Leaf<R> &L, R a, R b, R rho, R t_min, R t_max, R &t_lo)
 {
     if (square_outside<R>(T, L.slo + L.w * a, L.tlo + L.w * b, L.w * rho))
@@ -605,7 +577,7 @@ __device__ bool sub_square_bound(const TriRecord &T, const Leaf<R> &L, R a, R b,
     R hA = (L.ha + L.tw * b) * rho, hB = (L.hb + L.tw * a) * rho, hk = fabs(L.tw) * rho * rho;
     R hmag = fabs(hc) + fabs(hA) + fabs(hB) + hk;
     R F1c, F1a, F1b, F1k, F2c, F2a, F2b, F2k, Gc, Ga, Gb, Gk;
-    if (T.const_normal) {
+    if (CN || T.const_normal) {
         F1c = L.c1 + a * L.p1a + b * L.p1b + hc * L.n1;
         F1a = rho * L.p1a + hA * L.n1;
         F1b = rho * L.p1b + hB * L.n1;
@@ -657,13 +629,13 @@ __device__ bool sub_square_bound(const TriRecord &T, const Leaf<R> &L, R a, R b,
 
 // F = ((S - o) . d1, (S - o) . d2), the distance along the ray and the
 // Jacobian of F at texel-local (a, b) (displaced_intersector.cpp:337-375).
-template <class R>
+template <class R, bool CN>
 __device__ inline void newton_eval(const TriRecord &T, const Leaf<R> &L, R a, R b, R &F1, R &F2, R &along, R &J00,
                                    R &J01, R &J10, R &J11)
 {
     R h = ((L.h00 + L.ha * a) + L.hb * b) + L.tw * a * b;
     R hda = L.ha + L.tw * b, hdb = L.hb + L.tw * a;
-    if (T.const_normal) {
+    if (CN || T.const_normal) {
         F1 = L.c1 + a * L.p1a + b * L.p1b + h * L.n1;
         F2 = L.c2 + a * L.p2a + b * L.p2b + h * L.n2;
         along = L.cn + a * L.pna + b * L.pnb + h * L.nn;
@@ -699,7 +671,7 @@ __device__ inline void polish_root(const TriRecord &T, const Leaf<double> &L, do
 {
     for (int it = 0; it < 2; ++it) {
         double F1, F2, along, J00, J01, J10, J11;
-        newton_eval<double>(T, L, a, b, F1, F2, along, J00, J01, J10, J11);
+        newton_eval<double, false>(T, L, a, b, F1, F2, along, J00, J01, J10, J11);
         double det = J00 * J11 - J01 * J10;
         if (det == 0.0)
             return;
@@ -713,7 +685,7 @@ __device__ inline void polish_root(const TriRecord &T, const Leaf<double> &L, do
 
 // newton_from (displaced_intersector.cpp:325-388) in texel-local
 // coordinates. On success writes the root (a, b) and its t.
-template <class R>
+template <class R, bool CN>
 __device__ bool newton_from(const TriRecord &T, const Leaf<R> &L, R a, R b, R t_min, R t_max, R &out_a, R &out_b,
                             R &out_t, int &iters)
 {
@@ -722,7 +694,7 @@ __device__ bool newton_from(const TriRecord &T, const Leaf<R> &L, R a, R b, R t_
     for (int iter = 0; iter < MAX_ITERS; ++iter) {
         ++iters;
         R F1, F2, along, J00, J01, J10, J11;
-        newton_eval<R>(T, L, a, b, F1, F2, along, J00, J01, J10, J11);
+        newton_eval<R, CN>(T, L, a, b, F1, F2, along, J00, J01, J10, J11);
         R err2 = F1 * F1 + F2 * F2;
         err_streak = err2 > prev_err2 ? err_streak + 1 : 0;
         behind_streak = along < (R)0 ? behind_streak + 1 : 0;
@@ -778,7 +750,7 @@ struct LaneStats
 // over an explicit stack; entries are (depth, qa, qb) packed in 16 bits and
 // the sub-square's lower bound on t. Children are pushed farthest first,
 // so they pop nearest first, as the recursion visits them.
-template <class R>
+template <class R, bool CN>
 __device__ bool search_leaf(const TriRecord &T, const Leaf<R> &L, R t_min, R &t_max, R &hit_a, R &hit_b,
                             typename StackSlot<R>::T *s_t, uint16_t *s_code, LaneStats &st)
 {
@@ -792,7 +764,7 @@ __device__ bool search_leaf(const TriRecord &T, const Leaf<R> &L, R t_min, R &t_
             ++st.runs;
             R scale = (R)1 / (R)(1 << SUBDIV_DEPTH);
             R ra, rb, rt;
-            if (newton_from<R>(T, L, ((R)qa + (R)0.5) * scale, ((R)qb + (R)0.5) * scale, t_min, t_max, ra, rb, rt,
+            if (newton_from<R, CN>(T, L, ((R)qa + (R)0.5) * scale, ((R)qb + (R)0.5) * scale, t_min, t_max, ra, rb, rt,
                                st.iters)) {
                 hit_a = ra;
                 hit_b = rb;
@@ -810,7 +782,7 @@ __device__ bool search_leaf(const TriRecord &T, const Leaf<R> &L, R t_min, R &t_
                 cc[q] = ((uint32_t)cd << 10) | ((uint32_t)ca << 5) | (uint32_t)cb;
                 ++st.sub;
                 R tl;
-                bool keep = sub_square_bound<R>(T, L, ((R)ca + (R)0.5) * inv, ((R)cb + (R)0.5) * inv, rho, t_min,
+                bool keep = sub_square_bound<R, CN>(T, L, ((R)ca + (R)0.5) * inv, ((R)cb + (R)0.5) * inv, rho, t_min,
                                                t_max, tl);
                 key[q] = keep ? tl : (R)INFINITY;
             }
@@ -923,15 +895,170 @@ __device__ bool leaf_two_triangle(const TraceParams &p, const TriRecord &T, cons
 
 // ------------------------------------------------------------------ kernel
 
+// Box of a BVH node or a triangle, clipped against [t_min, t_max].
+template <class R>
+__device__ inline bool ray_box(const R o[3], const R d[3], const R inv[3], const float lo[3], const float hi[3], R t_min,
+                               R t_max)
+{
+    R t0 = t_min, t1 = t_max;
+    return clip_axis<R>(o[0], d[0], inv[0], (R)lo[0], (R)hi[0], t0, t1) &&
+           clip_axis<R>(o[1], d[1], inv[1], (R)lo[1], (R)hi[1], t0, t1) &&
+           clip_axis<R>(o[2], d[2], inv[2], (R)lo[2], (R)hi[2], t0, t1) && close_interval<R>(t0, t1);
+}
+
+template <class R> __device__ inline R axis_of(const R v[3], int a) { return a == 0 ? v[0] : a == 1 ? v[1] : v[2]; }
+
 enum
 {
-    ST_NEXT = 0,
-    ST_WALK = 1,
-    ST_LEAF = 2,
-    ST_DONE = 3
+    FROM_PARENT = 0,
+    FROM_SIBLING = 1,
+    FROM_CHILD = 2
 };
 
-template <class R, bool STATS>
+// Stackless traversal of the object's BVH (parent links, near child first
+// along the node's split axis): advances to the next triangle whose box
+// the ray meets within [t_min, t_max] and returns it, or -1 when the tree
+// is exhausted. The closest hit does not depend on the order triangles are
+// visited in (ObjectIntersector::intersect_flat keeps the nearest,
+// displaced_intersector.cpp:578-599); visiting near boxes first lets the
+// shrinking t_max cull the rest.
+template <class R>
+__device__ int bvh_next(const BvhNode *N, int &cur, int &from, const R o[3], const R d[3], const R inv[3], R t_min,
+                        R t_max)
+{
+    while (true) {
+        if (from == FROM_CHILD) {
+            if (cur == 0)
+                return -1;
+            const BvhNode &par = N[N[cur].parent];
+            int near = axis_of(d, par.axis) >= (R)0 ? par.left : par.right;
+            if (cur == near) {
+                cur = near == par.left ? par.right : par.left;
+                from = FROM_SIBLING;
+            } else {
+                cur = N[cur].parent;
+            }
+            continue;
+        }
+        const BvhNode &nd = N[cur];
+        if (!ray_box<R>(o, d, inv, nd.lo, nd.hi, t_min, t_max)) {
+            if (cur == 0)
+                return -1;
+            if (from == FROM_PARENT) {
+                const BvhNode &par = N[nd.parent];
+                cur = cur == par.left ? par.right : par.left;
+                from = FROM_SIBLING;
+            } else {
+                from = FROM_CHILD;
+            }
+            continue;
+        }
+        if (nd.left < 0) {
+            int tri = ~nd.left;
+            if (from == FROM_PARENT && cur != 0) {
+                const BvhNode &par = N[nd.parent];
+                cur = cur == par.left ? par.right : par.left;
+                from = FROM_SIBLING;
+            } else {
+                from = FROM_CHILD;
+            }
+            return tri;
+        }
+        cur = axis_of(d, nd.axis) >= (R)0 ? nd.left : nd.right;
+        from = FROM_PARENT;
+    }
+}
+
+// The four children of cell (pi, pj) at `level` + 1 share one Morton group
+// at `level`: their min-max pairs are one 32 B load, their Taylor quads one
+// 64 B load.
+struct Group
+{
+    float4 mm01, mm23;
+    float4 tp[4];
+};
+
+__device__ inline void load_group(const PyramidView &P, int level, int pi, int pj, bool want_tp, Group &g)
+{
+    if (level >= P.n_levels) {
+        float lo = P.root_hmin, hi = P.root_hmax;
+        g.mm01 = make_float4(lo, hi, lo, hi);
+        g.mm23 = g.mm01;
+        float4 t = make_float4(0.5f * (lo + hi), 0.f, 0.f, 0.5f * (hi - lo));
+        g.tp[0] = g.tp[1] = g.tp[2] = g.tp[3] = t;
+        return;
+    }
+    uint32_t m = (uint32_t)P.n_leaf >> level;
+    uint32_t base = level_base(m) + morton2((uint32_t)(2 * pi) & (m - 1), (uint32_t)(2 * pj) & (m - 1));
+    const float4 *mm4 = reinterpret_cast<const float4 *>(P.mm + base);
+    g.mm01 = mm4[0];
+    g.mm23 = mm4[1];
+    if (want_tp) {
+        const float4 *tp4 = P.tp + base;
+        g.tp[0] = tp4[0];
+        g.tp[1] = tp4[1];
+        g.tp[2] = tp4[2];
+        g.tp[3] = tp4[3];
+    }
+}
+
+__device__ inline float2 group_mm(const Group &g, int q)
+{
+    float4 v = q < 2 ? g.mm01 : g.mm23;
+    return (q & 1) ? make_float2(v.z, v.w) : make_float2(v.x, v.y);
+}
+__device__ inline float4 group_tp(const Group &g, int q)
+{
+    float4 a = (q & 1) ? g.tp[1] : g.tp[0], b = (q & 1) ? g.tp[3] : g.tp[2];
+    return q < 2 ? a : b;
+}
+
+// Node tests of the four children of (pi, pj) (children at `level`),
+// skipping those outside the domain (TFDM's texel discard, before any
+// node is used). Returns the passing children as a 4-bit mask in the order
+// the ray meets them: bit k is child (2 pi + ((k >> 1) ^ si), 2 pj + ((k & 1) ^ sj)),
+// the child order of cell_next (displaced_intersector.cpp:99-121).
+template <class R, bool CN>
+__device__ uint32_t expand(const TraceParams &p, const TriRecord &T, const Walk<R> &wk, int level, int pi, int pj,
+                           R t_min, R t_max, int &node_tests)
+{
+    const PyramidView &P = p.pyr;
+    bool slab = p.bound != 0 && level <= p.slab_max_level;
+    Group g;
+    load_group(P, level, pi, pj, slab, g);
+    R w = ldexp((R)1, level) / (R)P.n_leaf;
+    uint32_t mask = 0;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        int io = q & 1, jo = q >> 1;
+        int ci = 2 * pi + io, cj = 2 * pj + jo;
+        if (square_outside<R>(T, ((R)ci + (R)0.5) * w, ((R)cj + (R)0.5) * w, (R)0.5 * w))
+            continue;
+        ++node_tests;
+        float4 tp = slab ? group_tp(g, q) : make_float4(0.f, 0.f, 0.f, 0.f);
+        if (cell_test<R, CN>(p, T, wk, level, ci, cj, group_mm(g, q), tp, t_min, t_max))
+            mask |= 1u << (((io ^ wk.si) << 1) | (jo ^ wk.sj));
+    }
+    return mask;
+}
+
+enum
+{
+    ST_NEXT = 0, // find the next triangle
+    ST_ROOT = 1, // test the next root cell of the triangle
+    ST_WALK = 2, // pop the next child from the trail
+    ST_LEAF = 3, // a leaf test is pending
+    ST_DONE = 4
+};
+
+// Closest or any hit, one ray per lane. The walk of each triangle is the
+// reference's depth-first descent from its roots (DisplacedIntersector::
+// intersect, displaced_intersector.cpp:175-237) with the children of a
+// passing cell tested together: `trail` keeps, four bits per level, the
+// children still to visit, in the ray's order. Leaves are searched when
+// every lane of the wave has one pending or is done (while-while), so the
+// leaf search runs wave-wide.
+template <class R, bool STATS, bool CN>
 __global__ void __launch_bounds__(BLOCK) k_trace(TraceParams p)
 {
     __shared__ typename StackSlot<R>::T s_t[STACK * BLOCK];
@@ -952,48 +1079,54 @@ __global__ void __launch_bounds__(BLOCK) k_trace(TraceParams p)
         t_best = (R)p.rays.tmax[rid];
     }
     const R o_f[3] = {(R)o[0], (R)o[1], (R)o[2]}, d_f[3] = {(R)d[0], (R)d[1], (R)d[2]};
-    R inv_f[3] = {(R)1 / d_f[0], (R)1 / d_f[1], (R)1 / d_f[2]};
+    const R inv_f[3] = {(R)1 / d_f[0], (R)1 / d_f[1], (R)1 / d_f[2]};
 
     int state = valid ? ST_NEXT : ST_DONE;
-    int k = p.prim_mask >= 0 ? p.prim_mask - 1 : -1;
-    int k_end = p.prim_mask >= 0 ? p.prim_mask + 1 : p.n_tri;
+    int bvh_cur = 0, bvh_from = FROM_PARENT;
+    bool single_done = false;
+    int k = -1;
     int hit_tri = -1, hit_i = 0, hit_j = 0;
     R hit_a = 0, hit_b = 0;
     LaneStats st{0, 0, 0, 0, 0, 0};
     Walk<R> wk;
-    int root_level = 0, n_roots = 0;
     const TriRecord *T = p.tris;
+    int root_level = 0, n_roots = 0, r = 0;
+    int lvl = 0, pi = 0, pj = 0; // the group being walked: children of (pi, pj), at level lvl
+    uint64_t trail = 0;
+    int leaf_i = 0, leaf_j = 0, after_leaf = ST_WALK;
 
     while (true) {
-        // Walk until a leaf is pending or the ray is done.
-        while (state == ST_NEXT || state == ST_WALK) {
+        while (state < ST_LEAF) {
             if (state == ST_NEXT) {
-                // intersect_flat (displaced_intersector.cpp:578-599): the next
-                // triangle whose box the ray meets before the current hit.
-                ++k;
-                if (k >= k_end) {
+                if (p.prim_mask >= 0) {
+                    k = -1;
+                    if (!single_done) {
+                        single_done = true;
+                        const float *bx = p.boxes + 6 * p.prim_mask;
+                        if (ray_box<R>(o_f, d_f, inv_f, bx, bx + 3, t_min, t_best))
+                            k = p.prim_mask;
+                    }
+                } else {
+                    k = bvh_next<R>(p.bvh, bvh_cur, bvh_from, o_f, d_f, inv_f, t_min, t_best);
+                }
+                if (k < 0) {
                     state = ST_DONE;
                     break;
                 }
-                const float *bx = p.boxes + 6 * k;
-                R t0 = t_min, t1 = t_best;
-                if (!clip_axis<R>(o_f[0], d_f[0], inv_f[0], (R)bx[0], (R)bx[3], t0, t1) ||
-                    !clip_axis<R>(o_f[1], d_f[1], inv_f[1], (R)bx[1], (R)bx[4], t0, t1) ||
-                    !clip_axis<R>(o_f[2], d_f[2], inv_f[2], (R)bx[2], (R)bx[5], t0, t1) || !close_interval<R>(t0, t1))
-                    continue;
                 if (STATS)
                     ++st.tri;
                 T = p.tris + k;
-                // The ray in tangent space (DisplacedIntersector::intersect :178).
+                // The ray in tangent space (DisplacedIntersector::intersect :178),
+                // formed in double.
                 double rel[3] = {o[0] - T->q0[0], o[1] - T->q0[1], o[2] - T->q0[2]};
                 R omag = 0, dmag = 0;
-                for (int r = 0; r < 3; ++r) {
-                    const double *row = T->T + 3 * r;
-                    wk.o[r] = (R)(row[0] * rel[0] + row[1] * rel[1] + row[2] * rel[2]);
-                    wk.d[r] = (R)(row[0] * d[0] + row[1] * d[1] + row[2] * d[2]);
-                    wk.inv[r] = (R)1 / wk.d[r];
-                    omag = fmax(omag, fabs(wk.o[r]));
-                    dmag = fmax(dmag, fabs(wk.d[r]));
+                for (int q = 0; q < 3; ++q) {
+                    const double *row = T->T + 3 * q;
+                    wk.o[q] = (R)(row[0] * rel[0] + row[1] * rel[1] + row[2] * rel[2]);
+                    wk.d[q] = (R)(row[0] * d[0] + row[1] * d[1] + row[2] * d[2]);
+                    wk.inv[q] = (R)1 / wk.d[q];
+                    omag = fmax(omag, fabs(wk.o[q]));
+                    dmag = fmax(dmag, fabs(wk.d[q]));
                 }
                 wk.omag = omag;
                 wk.dmag = dmag;
@@ -1001,78 +1134,103 @@ __global__ void __launch_bounds__(BLOCK) k_trace(TraceParams p)
                 wk.sj = wk.d[1] < (R)0;
                 root_level = T->root_level;
                 n_roots = T->n_roots;
-                wk.r = 0;
-                wk.level = root_level;
-                wk.i = T->root_i[0];
-                wk.j = T->root_j[0];
-                wk.elevel = root_level;
-                wk.ei = wk.i;
-                wk.ej = wk.j;
-                cell_next(wk.elevel, wk.ei, wk.ej, wk.si, wk.sj, root_level);
-                state = ST_WALK;
+                r = 0;
+                state = ST_ROOT;
                 continue;
             }
-            // ST_WALK
-            if (wk.level == wk.elevel && wk.i == wk.ei && wk.j == wk.ej) {
-                if (++wk.r >= n_roots) {
+            if (state == ST_ROOT) {
+                if (r >= n_roots) {
                     state = ST_NEXT;
                     continue;
                 }
-                wk.level = root_level;
-                wk.i = T->root_i[wk.r];
-                wk.j = T->root_j[wk.r];
-                wk.elevel = root_level;
-                wk.ei = wk.i;
-                wk.ej = wk.j;
-                cell_next(wk.elevel, wk.ei, wk.ej, wk.si, wk.sj, root_level);
+                int ri = T->root_i[r], rj = T->root_j[r];
+                ++r;
+                R w = ldexp((R)1, root_level) / (R)p.pyr.n_leaf;
+                if (square_outside<R>(*T, ((R)ri + (R)0.5) * w, ((R)rj + (R)0.5) * w, (R)0.5 * w))
+                    continue;
+                if (STATS)
+                    ++st.node;
+                bool slab = p.bound != 0 && root_level <= p.slab_max_level;
+                float4 tp = slab ? fetch_tp(p.pyr, root_level, ri, rj) : make_float4(0.f, 0.f, 0.f, 0.f);
+                if (!cell_test<R, CN>(p, *T, wk, root_level, ri, rj, fetch_mm(p.pyr, root_level, ri, rj), tp, t_min,
+                                      t_best))
+                    continue;
+                if (root_level == 0) {
+                    leaf_i = ri;
+                    leaf_j = rj;
+                    after_leaf = ST_ROOT;
+                    state = ST_LEAF;
+                    break;
+                }
+                int tests = 0;
+                uint32_t mask = expand<R, CN>(p, *T, wk, root_level - 1, ri, rj, t_min, t_best, tests);
+                if (STATS)
+                    st.node += tests;
+                if (mask) {
+                    lvl = root_level - 1;
+                    pi = ri;
+                    pj = rj;
+                    trail = (uint64_t)mask << (4 * lvl);
+                    state = ST_WALK;
+                }
                 continue;
             }
-            R w = ldexp((R)1, wk.level) / (R)p.pyr.n_leaf;
-            if (square_outside<R>(*T, ((R)wk.i + (R)0.5) * w, ((R)wk.j + (R)0.5) * w, (R)0.5 * w)) {
-                cell_next(wk.level, wk.i, wk.j, wk.si, wk.sj, root_level);
+            // ST_WALK: the next child of the deepest group with one left.
+            uint32_t m = (uint32_t)(trail >> (4 * lvl)) & 15u;
+            if (m == 0) {
+                if (lvl >= root_level - 1) {
+                    state = ST_ROOT;
+                    continue;
+                }
+                ++lvl;
+                pi >>= 1;
+                pj >>= 1;
                 continue;
             }
+            int kk = __ffs(m) - 1;
+            trail &= ~((uint64_t)1 << (4 * lvl + kk));
+            int ci = 2 * pi + ((kk >> 1) ^ wk.si), cj = 2 * pj + ((kk & 1) ^ wk.sj);
+            if (lvl == 0) {
+                if (STATS)
+                    ++st.leaf;
+                leaf_i = ci;
+                leaf_j = cj;
+                after_leaf = ST_WALK;
+                state = ST_LEAF;
+                break;
+            }
+            int tests = 0;
+            uint32_t mask = expand<R, CN>(p, *T, wk, lvl - 1, ci, cj, t_min, t_best, tests);
             if (STATS)
-                ++st.node;
-            if (!node_test<R>(p, *T, wk, t_min, t_best)) {
-                cell_next(wk.level, wk.i, wk.j, wk.si, wk.sj, root_level);
-                continue;
+                st.node += tests;
+            if (mask) {
+                --lvl;
+                pi = ci;
+                pj = cj;
+                trail |= (uint64_t)mask << (4 * lvl);
             }
-            if (wk.level > 0) {
-                // cell_down (displaced_intersector.cpp:92-97)
-                --wk.level;
-                wk.i = 2 * wk.i + wk.si;
-                wk.j = 2 * wk.j + wk.sj;
-                continue;
-            }
-            if (STATS)
-                ++st.leaf;
-            state = ST_LEAF;
         }
         if (__all(state == ST_DONE))
             break;
         if (state == ST_LEAF) {
+            if (STATS && after_leaf == ST_ROOT)
+                ++st.leaf;
             Leaf<R> L;
-            leaf_setup<R>(p, *T, o, d, wk.i, wk.j, L);
+            leaf_setup<R>(p, *T, o, d, leaf_i, leaf_j, L);
             R la = 0, lb = 0;
             bool found;
             if (p.leaf == 0)
-                found = search_leaf<R>(*T, L, t_min, t_best, la, lb, s_t, s_code, st);
+                found = search_leaf<R, CN>(*T, L, t_min, t_best, la, lb, s_t, s_code, st);
             else
                 found = leaf_two_triangle<R>(p, *T, L, o, d, t_min, t_best, la, lb);
             if (found) {
                 hit_tri = k;
-                hit_i = wk.i;
-                hit_j = wk.j;
+                hit_i = leaf_i;
+                hit_j = leaf_j;
                 hit_a = la;
                 hit_b = lb;
             }
-            if (found && p.any_hit) {
-                state = ST_DONE;
-            } else {
-                cell_next(wk.level, wk.i, wk.j, wk.si, wk.sj, root_level);
-                state = ST_WALK;
-            }
+            state = (found && p.any_hit) ? ST_DONE : after_leaf;
         }
     }
 
@@ -1081,118 +1239,14 @@ __global__ void __launch_bounds__(BLOCK) k_trace(TraceParams p)
     if (p.any_hit) {
         p.occluded[rid] = hit_tri >= 0 ? 1 : 0;
     } else {
+        // The raw record; k_finalize turns it into the hit.
         const HitsDev &H = p.hits;
-        if (hit_tri < 0) {
-            H.t[rid] = INFINITY;
-            if (H.prim)
-                H.prim[rid] = -1;
-            if (H.s)
-                H.s[rid] = 0.f;
-            if (H.tt)
-                H.tt[rid] = 0.f;
-            if (H.bu)
-                H.bu[rid] = 0.f;
-            if (H.bv)
-                H.bv[rid] = 0.f;
-            if (H.nx)
-                H.nx[rid] = 0.f;
-            if (H.ny)
-                H.ny[rid] = 0.f;
-            if (H.nz)
-                H.nz[rid] = 0.f;
-            if (H.texel_i)
-                H.texel_i[rid] = 0;
-            if (H.texel_j)
-                H.texel_j[rid] = 0;
-        } else {
-            // The reported hit, in double: the root polished (Newton leaf) or the
-            // face crossing recomputed (two-triangle leaf), then st, the chart's
-            // barycentrics and ng = unit (Su x Sv) at the root
-            // (displaced_intersector.cpp:357-366, 413-421).
-            const TriRecord &TH = p.tris[hit_tri];
-            Leaf<double> L;
-            leaf_setup<double>(p, TH, o, d, hit_i, hit_j, L);
-            double a = (double)hit_a, b = (double)hit_b, t_hit = (double)t_best;
-            double Sa[3], Sb[3];
-            if (p.leaf == 1) {
-                double ra, rb, tm = (double)INFINITY;
-                if (leaf_two_triangle<double>(p, TH, L, o, d, (double)t_min, tm, ra, rb)) {
-                    a = ra;
-                    b = rb;
-                    t_hit = tm;
-                }
-                bool upper = a + b > 1.0;
-                double hc[4] = {L.h00, L.h00 + L.ha, L.h00 + L.hb, L.h00 + L.ha + L.hb + L.tw};
-                double N[3], Na[3], Nb[3], Sc[4][3];
-                const double ca[4] = {0, 1, 0, 1}, cb[4] = {0, 0, 1, 1};
-                for (int c = 0; c < 4; ++c) {
-                    if (TH.const_normal) {
-                        for (int q = 0; q < 3; ++q)
-                            N[q] = L.N[q];
-                    } else {
-                        normal_at(L, ca[c], cb[c], N, Na, Nb);
-                    }
-                    for (int q = 0; q < 3; ++q)
-                        Sc[c][q] = ca[c] * L.E1[q] + cb[c] * L.E2[q] + hc[c] * N[q];
-                }
-                int A = upper ? 1 : 0, B = upper ? 3 : 1, C = 2;
-                for (int q = 0; q < 3; ++q) {
-                    Sa[q] = Sc[B][q] - Sc[A][q];
-                    Sb[q] = Sc[C][q] - Sc[A][q];
-                }
-            } else {
-                polish_root(TH, L, a, b);
-                double F1, F2, along, J00, J01, J10, J11;
-                newton_eval<double>(TH, L, a, b, F1, F2, along, J00, J01, J10, J11);
-                t_hit = along / L.d_len;
-                double h = ((L.h00 + L.ha * a) + L.hb * b) + L.tw * a * b;
-                double hda = L.ha + L.tw * b, hdb = L.hb + L.tw * a;
-                if (TH.const_normal) {
-                    for (int q = 0; q < 3; ++q) {
-                        Sa[q] = L.E1[q] + hda * L.N[q];
-                        Sb[q] = L.E2[q] + hdb * L.N[q];
-                    }
-                } else {
-                    double N[3], Na[3], Nb[3];
-                    normal_at(L, a, b, N, Na, Nb);
-                    for (int q = 0; q < 3; ++q) {
-                        Sa[q] = L.E1[q] + hda * N[q] + h * Na[q];
-                        Sb[q] = L.E2[q] + hdb * N[q] + h * Nb[q];
-                    }
-                }
-            }
-            double s = L.slo + L.w * a, t = L.tlo + L.w * b;
-            double n[3] = {Sa[1] * Sb[2] - Sa[2] * Sb[1], Sa[2] * Sb[0] - Sa[0] * Sb[2],
-                           Sa[0] * Sb[1] - Sa[1] * Sb[0]};
-            double len = sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
-            if (len > 0.0)
-                for (int q = 0; q < 3; ++q)
-                    n[q] /= len;
-            H.t[rid] = (float)t_hit;
-            if (H.prim)
-                H.prim[rid] = hit_tri;
-            if (H.s)
-                H.s[rid] = (float)s;
-            if (H.tt)
-                H.tt[rid] = (float)t;
-            if (H.bu || H.bv) {
-                double ds = s - TH.t0[0], dt = t - TH.t0[1];
-                if (H.bu)
-                    H.bu[rid] = (float)(TH.J[0] * ds + TH.J[1] * dt);
-                if (H.bv)
-                    H.bv[rid] = (float)(TH.J[2] * ds + TH.J[3] * dt);
-            }
-            if (H.nx)
-                H.nx[rid] = (float)n[0];
-            if (H.ny)
-                H.ny[rid] = (float)n[1];
-            if (H.nz)
-                H.nz[rid] = (float)n[2];
-            if (H.texel_i)
-                H.texel_i[rid] = hit_i;
-            if (H.texel_j)
-                H.texel_j[rid] = hit_j;
-        }
+        H.t[rid] = hit_tri >= 0 ? (float)t_best : INFINITY;
+        H.prim[rid] = hit_tri;
+        H.texel_i[rid] = hit_i;
+        H.texel_j[rid] = hit_j;
+        H.s[rid] = (float)hit_a;
+        H.tt[rid] = (float)hit_b;
     }
     if (STATS) {
         const StatsDev &S = p.stats;
@@ -1209,6 +1263,112 @@ __global__ void __launch_bounds__(BLOCK) k_trace(TraceParams p)
         if (S.newton_iters)
             S.newton_iters[rid] = st.iters;
     }
+}
+
+// The reported hit, in double: the root polished (Newton leaf) or the face
+// crossing recomputed (two-triangle leaf), then st, the chart's
+// barycentrics and ng = unit (Su x Sv) at the root
+// (displaced_intersector.cpp:357-366, 413-421).
+__global__ void __launch_bounds__(256) k_finalize(TraceParams p)
+{
+    int64_t rid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (rid >= p.n_rays)
+        return;
+    const HitsDev &H = p.hits;
+    int tri = H.prim[rid];
+    if (tri < 0) {
+        H.s[rid] = 0.f;
+        H.tt[rid] = 0.f;
+        H.texel_i[rid] = 0;
+        H.texel_j[rid] = 0;
+        if (H.bu)
+            H.bu[rid] = 0.f;
+        if (H.bv)
+            H.bv[rid] = 0.f;
+        if (H.nx)
+            H.nx[rid] = 0.f;
+        if (H.ny)
+            H.ny[rid] = 0.f;
+        if (H.nz)
+            H.nz[rid] = 0.f;
+        return;
+    }
+    double o[3] = {p.rays.ox[rid], p.rays.oy[rid], p.rays.oz[rid]};
+    double d[3] = {p.rays.dx[rid], p.rays.dy[rid], p.rays.dz[rid]};
+    double t_min = p.rays.tmin[rid];
+    const TriRecord &TH = p.tris[tri];
+    int hi = H.texel_i[rid], hj = H.texel_j[rid];
+    Leaf<double> L;
+    leaf_setup<double>(p, TH, o, d, hi, hj, L);
+    double a = H.s[rid], b = H.tt[rid], t_hit = H.t[rid];
+    double Sa[3], Sb[3];
+    if (p.leaf == 1) {
+        double ra, rb, tm = (double)INFINITY;
+        if (leaf_two_triangle<double>(p, TH, L, o, d, t_min, tm, ra, rb)) {
+            a = ra;
+            b = rb;
+            t_hit = tm;
+        }
+        bool upper = a + b > 1.0;
+        double hc[4] = {L.h00, L.h00 + L.ha, L.h00 + L.hb, L.h00 + L.ha + L.hb + L.tw};
+        double N[3], Na[3], Nb[3], Sc[4][3];
+        const double ca[4] = {0, 1, 0, 1}, cb[4] = {0, 0, 1, 1};
+        for (int c = 0; c < 4; ++c) {
+            if (TH.const_normal) {
+                for (int q = 0; q < 3; ++q)
+                    N[q] = L.N[q];
+            } else {
+                normal_at(L, ca[c], cb[c], N, Na, Nb);
+            }
+            for (int q = 0; q < 3; ++q)
+                Sc[c][q] = ca[c] * L.E1[q] + cb[c] * L.E2[q] + hc[c] * N[q];
+        }
+        int A = upper ? 1 : 0, B = upper ? 3 : 1, C = 2;
+        for (int q = 0; q < 3; ++q) {
+            Sa[q] = Sc[B][q] - Sc[A][q];
+            Sb[q] = Sc[C][q] - Sc[A][q];
+        }
+    } else {
+        polish_root(TH, L, a, b);
+        double F1, F2, along, J00, J01, J10, J11;
+        newton_eval<double, false>(TH, L, a, b, F1, F2, along, J00, J01, J10, J11);
+        t_hit = along / L.d_len;
+        double h = ((L.h00 + L.ha * a) + L.hb * b) + L.tw * a * b;
+        double hda = L.ha + L.tw * b, hdb = L.hb + L.tw * a;
+        if (TH.const_normal) {
+            for (int q = 0; q < 3; ++q) {
+                Sa[q] = L.E1[q] + hda * L.N[q];
+                Sb[q] = L.E2[q] + hdb * L.N[q];
+            }
+        } else {
+            double N[3], Na[3], Nb[3];
+            normal_at(L, a, b, N, Na, Nb);
+            for (int q = 0; q < 3; ++q) {
+                Sa[q] = L.E1[q] + hda * N[q] + h * Na[q];
+                Sb[q] = L.E2[q] + hdb * N[q] + h * Nb[q];
+            }
+        }
+    }
+    double s = L.slo + L.w * a, t = L.tlo + L.w * b;
+    double n[3] = {Sa[1] * Sb[2] - Sa[2] * Sb[1], Sa[2] * Sb[0] - Sa[0] * Sb[2], Sa[0] * Sb[1] - Sa[1] * Sb[0]};
+    double len = sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+    if (len > 0.0)
+        for (int q = 0; q < 3; ++q)
+            n[q] /= len;
+    H.t[rid] = (float)t_hit;
+    H.s[rid] = (float)s;
+    H.tt[rid] = (float)t;
+    double ds = s - TH.t0[0], dt = t - TH.t0[1];
+    if (H.bu)
+        H.bu[rid] = (float)(TH.J[0] * ds + TH.J[1] * dt);
+    if (H.bv)
+        H.bv[rid] = (float)(TH.J[2] * ds + TH.J[3] * dt);
+    if (H.nx)
+        H.nx[rid] = (float)n[0];
+    if (H.ny)
+        H.ny[rid] = (float)n[1];
+    if (H.nz)
+        H.nz[rid] = (float)n[2];
 }
 
 // ------------------------------------------------------- height range
@@ -1280,6 +1440,15 @@ hipError_t launch_height_range(const PyramidView &P, const TriRecord *tris, int 
     return hipGetLastError();
 }
 
+template <class R, bool STATS>
+static void launch_trace_r(const TraceParams &p, unsigned blocks, hipStream_t stream)
+{
+    if (p.const_normal)
+        hipLaunchKernelGGL((k_trace<R, STATS, true>), dim3(blocks), dim3(BLOCK), 0, stream, p);
+    else
+        hipLaunchKernelGGL((k_trace<R, STATS, false>), dim3(blocks), dim3(BLOCK), 0, stream, p);
+}
+
 hipError_t launch_trace(const TraceParams &p, int precision, hipStream_t stream)
 {
     if (p.n_rays <= 0)
@@ -1289,15 +1458,20 @@ hipError_t launch_trace(const TraceParams &p, int precision, hipStream_t stream)
                  p.stats.newton_runs || p.stats.newton_iters;
     if (precision == 1) {
         if (stats)
-            hipLaunchKernelGGL((k_trace<double, true>), dim3(blocks), dim3(BLOCK), 0, stream, p);
+            launch_trace_r<double, true>(p, blocks, stream);
         else
-            hipLaunchKernelGGL((k_trace<double, false>), dim3(blocks), dim3(BLOCK), 0, stream, p);
+            launch_trace_r<double, false>(p, blocks, stream);
     } else {
         if (stats)
-            hipLaunchKernelGGL((k_trace<float, true>), dim3(blocks), dim3(BLOCK), 0, stream, p);
+            launch_trace_r<float, true>(p, blocks, stream);
         else
-            hipLaunchKernelGGL((k_trace<float, false>), dim3(blocks), dim3(BLOCK), 0, stream, p);
+            launch_trace_r<float, false>(p, blocks, stream);
     }
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess || p.any_hit)
+        return e;
+    unsigned fblocks = (unsigned)((p.n_rays + 255) / 256);
+    hipLaunchKernelGGL(k_finalize, dim3(fblocks), dim3(256), 0, stream, p);
     return hipGetLastError();
 }
 

@@ -66,7 +66,8 @@ typedef struct
     const float *ox, *oy, *oz, *dx, *dy, *dz, *tmin, *tmax;
 } dmap_rays;
 
-/* Device pointers, n entries each; every field but t may be NULL. */
+/* Device pointers, n entries each; t, s, tt, prim, texel_i and texel_j
+ * are required, bu, bv, nx, ny, nz may be NULL. */
 typedef struct
 {
     float *t, *s, *tt, *bu, *bv, *nx, *ny, *nz;
@@ -111,6 +112,8 @@ void dmap_tile_destroy(dmap_tile *tile);
  * (t0 t1 t2). Host pointers. */
 int dmap_object_create(dmap_ctx *ctx, dmap_tile *tile, int n_tri, const double *positions, const double *normals,
                        const double *uvs, dmap_object **out);
+/* The top-level BVH over the triangles' AABBs stands in for the reference's
+ * embree scene (ObjectIntersector::build_bvh, displaced_intersector.cpp:540-554). */
 /* Per-triangle set-up: root level, root count, roots (4 x (i, j)), the
  * object-space AABB (lo xyz, hi xyz, outward-rounded fp32) and the height
  * range over the footprint. */
