@@ -49,6 +49,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-sample", type=int, default=4, help="oracle sample: every k-th ray")
     ap.add_argument("--precision", default="fp32", choices=["fp32", "fp64"])
+    ap.add_argument("--no-reorder", action="store_true", help="trace the rays in their given order")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus}); "
@@ -68,7 +69,7 @@ def main():
     obj = dmap_amd.ObjectIntersector(tile, wl.triangles)
     rays = torch.from_numpy(wl.rays).cuda()
     hits = obj.alloc_hits(n)
-    opt = dmap_amd.TraversalOptions(precision=args.precision)
+    opt = dmap_amd.TraversalOptions(precision=args.precision, reorder=not args.no_reorder)
     stream = torch.cuda.current_stream()
     torch.cuda.synchronize()
 

@@ -44,7 +44,7 @@ class DmapError(RuntimeError):
 
 class _Options(ctypes.Structure):
     _fields_ = [("bound", ctypes.c_int), ("slab_max_level", ctypes.c_int), ("leaf", ctypes.c_int),
-                ("precision", ctypes.c_int)]
+                ("precision", ctypes.c_int), ("reorder", ctypes.c_int)]
 
 
 _FP = ctypes.c_void_p
@@ -121,9 +121,11 @@ class TraversalOptions:
     slab_max_level: int = 1
     leaf: str = "newton"
     precision: str = "fp32"
+    reorder: bool = True
 
     def _c(self):
-        return _Options(BOUND[self.bound], int(self.slab_max_level), LEAF[self.leaf], PRECISION[self.precision])
+        return _Options(BOUND[self.bound], int(self.slab_max_level), LEAF[self.leaf], PRECISION[self.precision],
+                        1 if self.reorder else 0)
 
 
 class Context:

@@ -46,6 +46,7 @@ struct dmap_object
     dmap_tile *tile = nullptr;
     int n_tri = 0;
     int const_normal = 0;
+    float bound_lo[3] = {0, 0, 0}, bound_hi[3] = {0, 0, 0};
     std::vector<TriRecord> host;
     std::vector<float> boxes_host, h_range_host;
     std::vector<BvhNode> bvh_host;
@@ -692,6 +693,10 @@ int dmap_object_create(dmap_ctx *ctx, dmap_tile *tile, int n_tri, const double *
     o->const_normal = 1;
     for (int k = 0; k < n_tri; ++k)
         o->const_normal &= o->host[k].const_normal;
+    for (int a = 0; a < 3; ++a) {
+        o->bound_lo[a] = o->bvh_host[0].lo[a];
+        o->bound_hi[a] = o->bvh_host[0].hi[a];
+    }
     e = hipMalloc(&o->bvh, sizeof(BvhNode) * o->bvh_host.size());
     if (e == hipSuccess)
         e = hipMemcpy(o->bvh, o->bvh_host.data(), sizeof(BvhNode) * o->bvh_host.size(), hipMemcpyHostToDevice);
@@ -740,7 +745,7 @@ void dmap_object_destroy(dmap_object *obj)
 
 static int check_options(const dmap_options *opt, dmap_options &o)
 {
-    o = dmap_options{2, 1, 0, 0};
+    o = dmap_options{2, 1, 0, 0, 1};
     if (opt)
         o = *opt;
     if (o.bound < 0 || o.bound > 2 || o.leaf < 0 || o.leaf > 1 || o.precision < 0 || o.precision > 1)
@@ -789,7 +794,29 @@ static int trace(const dmap_object *obj, const dmap_rays *rays, int64_t n, const
     if (stats)
         p.stats = StatsDev{stats->triangle_tests, stats->node_tests, stats->leaf_tests, stats->sub_squares,
                            stats->newton_runs, stats->newton_iters};
-    HIP_TRY(launch_trace(p, o.precision, (hipStream_t)stream));
+    hipStream_t s = (hipStream_t)stream;
+    if (n > 0x7fffffffLL)
+        return fail(DMAP_ERR_INVALID, "intersect: at most 2^31 - 1 rays per call");
+    // Per-call workspace, stream-ordered: the ray queue head, and the
+    // reorder buffers when asked for.
+    bool reorder = o.reorder && n >= 4096;
+    size_t rbytes = reorder ? reorder_workspace_bytes(n) : 0;
+    void *ws = nullptr;
+    HIP_TRY(hipMallocAsync(&ws, 256 + rbytes, s));
+    p.queue = (unsigned *)ws;
+    if (reorder) {
+        uint32_t *perm = nullptr;
+        hipError_t e = reorder_rays(p.rays, n, obj->bound_lo, obj->bound_hi, (char *)ws + 256, rbytes, &perm, s);
+        if (e != hipSuccess) {
+            hipFreeAsync(ws, s);
+            return fail(DMAP_ERR_HIP, "intersect: ray reorder: %s", hipGetErrorString(e));
+        }
+        p.perm = perm;
+    }
+    hipError_t e = launch_trace(p, o.precision, s);
+    if (ws)
+        hipFreeAsync(ws, s);
+    HIP_TRY(e);
     return DMAP_OK;
 }
 

@@ -67,11 +67,19 @@ struct TraceParams
     int leaf; // 0 newton, 1 two-triangle
     int any_hit;
     int64_t n_rays;
+    const uint32_t *perm; // lane order of the rays (NULL: as given)
+    unsigned *queue;      // the launch's ray queue head (reset by launch_trace)
     RaysDev rays;
     HitsDev hits;
     uint8_t *occluded;
     StatsDev stats;
 };
+
+// Sort rays for coherence (reorder.hip): writes a permutation of 0..n-1
+// into the workspace and returns it in *perm.
+size_t reorder_workspace_bytes(int64_t n);
+hipError_t reorder_rays(const RaysDev &rays, int64_t n, const float lo[3], const float hi[3], void *workspace,
+                        size_t workspace_bytes, uint32_t **perm, hipStream_t stream);
 
 // Closest hit / any hit over the object; precision 0 = fp32, 1 = fp64.
 // A closest-hit launch is followed by the finalize kernel, which turns the

@@ -1013,29 +1013,47 @@ __device__ inline float4 group_tp(const Group &g, int q)
     return q < 2 ? a : b;
 }
 
-// Node tests of the four children of (pi, pj) (children at `level`),
-// skipping those outside the domain (TFDM's texel discard, before any
-// node is used). Returns the passing children as a 4-bit mask in the order
-// the ray meets them: bit k is child (2 pi + ((k >> 1) ^ si), 2 pj + ((k & 1) ^ sj)),
-// the child order of cell_next (displaced_intersector.cpp:99-121).
+// Node tests of a 2 x 2 block of cells at `level` with lowest corner
+// (bi, bj), skipping cells outside the domain (TFDM's texel discard, before
+// any node is used) and cells not in `valid` (Morton bit q = io + 2 jo).
+// An aligned block is the four children of one cell, one Morton group; the
+// roots of a triangle need not be aligned and are loaded one by one.
+// Returns the passing cells as a 4-bit mask in the order the ray meets
+// them: bit k is cell (bi + ((k >> 1) ^ si), bj + ((k & 1) ^ sj)), the
+// child order of cell_next (displaced_intersector.cpp:99-121).
 template <class R, bool CN>
-__device__ uint32_t expand(const TraceParams &p, const TriRecord &T, const Walk<R> &wk, int level, int pi, int pj,
-                           R t_min, R t_max, int &node_tests)
+__device__ uint32_t expand(const TraceParams &p, const TriRecord &T, const Walk<R> &wk, int level, int bi, int bj,
+                           bool aligned, uint32_t valid, R t_min, R t_max, int &node_tests)
 {
     const PyramidView &P = p.pyr;
     bool slab = p.bound != 0 && level <= p.slab_max_level;
     Group g;
-    load_group(P, level, pi, pj, slab, g);
+    if (aligned) {
+        load_group(P, level, bi >> 1, bj >> 1, slab, g);
+    } else {
+        float2 m0 = fetch_mm(P, level, bi, bj), m1 = fetch_mm(P, level, bi + 1, bj);
+        float2 m2 = fetch_mm(P, level, bi, bj + 1), m3 = fetch_mm(P, level, bi + 1, bj + 1);
+        g.mm01 = make_float4(m0.x, m0.y, m1.x, m1.y);
+        g.mm23 = make_float4(m2.x, m2.y, m3.x, m3.y);
+        if (slab) {
+            g.tp[0] = fetch_tp(P, level, bi, bj);
+            g.tp[1] = fetch_tp(P, level, bi + 1, bj);
+            g.tp[2] = fetch_tp(P, level, bi, bj + 1);
+            g.tp[3] = fetch_tp(P, level, bi + 1, bj + 1);
+        }
+    }
     R w = ldexp((R)1, level) / (R)P.n_leaf;
     uint32_t mask = 0;
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
         int io = q & 1, jo = q >> 1;
-        int ci = 2 * pi + io, cj = 2 * pj + jo;
+        int ci = bi + io, cj = bj + jo;
+        if (!((valid >> q) & 1u))
+            continue;
         if (square_outside<R>(T, ((R)ci + (R)0.5) * w, ((R)cj + (R)0.5) * w, (R)0.5 * w))
             continue;
         ++node_tests;
-        float4 tp = slab ? group_tp(g, q) : make_float4(0.f, 0.f, 0.f, 0.f);
+        float4 tp = slab ? g.tp[q] : make_float4(0.f, 0.f, 0.f, 0.f);
         if (cell_test<R, CN>(p, T, wk, level, ci, cj, group_mm(g, q), tp, t_min, t_max))
             mask |= 1u << (((io ^ wk.si) << 1) | (jo ^ wk.sj));
     }
@@ -1044,44 +1062,45 @@ __device__ uint32_t expand(const TraceParams &p, const TriRecord &T, const Walk<
 
 enum
 {
-    ST_NEXT = 0, // find the next triangle
-    ST_ROOT = 1, // test the next root cell of the triangle
-    ST_WALK = 2, // pop the next child from the trail
-    ST_LEAF = 3, // a leaf test is pending
-    ST_DONE = 4
+    ST_NEED = 0,   // bookkeeping: find the next block to expand or leaf to test
+    ST_EXPAND = 1, // a block's node tests are pending
+    ST_LEAF = 2,   // a leaf test is pending
+    ST_DONE = 3
 };
+
+// Leaves run when at least this share (in 1/64) of the wave's busy lanes
+// have one pending, or when no lane has a block to expand.
+constexpr int LEAF_BATCH = 24;
+// A wave claims new rays when at least this many of its lanes are idle.
+constexpr int REFILL = 16;
 
 // Closest or any hit, one ray per lane. The walk of each triangle is the
 // reference's depth-first descent from its roots (DisplacedIntersector::
-// intersect, displaced_intersector.cpp:175-237) with the children of a
+// intersect, displaced_intersector.cpp:175-237) with the four children of a
 // passing cell tested together: `trail` keeps, four bits per level, the
-// children still to visit, in the ray's order. Leaves are searched when
-// every lane of the wave has one pending or is done (while-while), so the
-// leaf search runs wave-wide.
+// cells still to visit, in the ray's order. Each round of the outer loop
+// first advances every lane's bookkeeping (pops, climbs, the next triangle
+// from the BVH) to its next block of node tests, then runs those node
+// tests for the whole wave at once; pending leaves are searched together
+// once enough lanes have one.
 template <class R, bool STATS, bool CN>
 __global__ void __launch_bounds__(BLOCK) k_trace(TraceParams p)
 {
     __shared__ typename StackSlot<R>::T s_t[STACK * BLOCK];
     __shared__ uint16_t s_code[STACK * BLOCK];
 
-    int64_t rid = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
-    bool valid = rid < p.n_rays;
+    const int lane = threadIdx.x & 63;
+    // Persistent waves: a lane that has finished its ray writes the result
+    // and, once enough lanes of its wave are idle, the wave claims that many
+    // new rays from the launch's queue; the wave exits when the queue is
+    // drained and its lanes are done.
+    bool drained = false, have = false;
+    int64_t rid = 0;
     double o[3] = {0, 0, 0}, d[3] = {0, 0, 1};
     R t_min = 0, t_best = 0;
-    if (valid) {
-        o[0] = p.rays.ox[rid];
-        o[1] = p.rays.oy[rid];
-        o[2] = p.rays.oz[rid];
-        d[0] = p.rays.dx[rid];
-        d[1] = p.rays.dy[rid];
-        d[2] = p.rays.dz[rid];
-        t_min = (R)p.rays.tmin[rid];
-        t_best = (R)p.rays.tmax[rid];
-    }
-    const R o_f[3] = {(R)o[0], (R)o[1], (R)o[2]}, d_f[3] = {(R)d[0], (R)d[1], (R)d[2]};
-    const R inv_f[3] = {(R)1 / d_f[0], (R)1 / d_f[1], (R)1 / d_f[2]};
-
-    int state = valid ? ST_NEXT : ST_DONE;
+    R o_f[3], d_f[3], inv_f[3];
+    int state = ST_DONE;
+    bool walking = false;
     int bvh_cur = 0, bvh_from = FROM_PARENT;
     bool single_done = false;
     int k = -1;
@@ -1090,179 +1109,220 @@ __global__ void __launch_bounds__(BLOCK) k_trace(TraceParams p)
     LaneStats st{0, 0, 0, 0, 0, 0};
     Walk<R> wk;
     const TriRecord *T = p.tris;
-    int root_level = 0, n_roots = 0, r = 0;
-    int lvl = 0, pi = 0, pj = 0; // the group being walked: children of (pi, pj), at level lvl
+    int root_level = 0, rb_i = 0, rb_j = 0;
+    int lvl = 0, pi = 0, pj = 0; // the group being walked: the cells at lvl below (pi, pj), or the root block
     uint64_t trail = 0;
-    int leaf_i = 0, leaf_j = 0, after_leaf = ST_WALK;
+    int ex_level = 0, ex_i = 0, ex_j = 0; // the block to expand
+    bool ex_aligned = true;
+    uint32_t ex_valid = 15;
+    int leaf_i = 0, leaf_j = 0;
 
     while (true) {
-        while (state < ST_LEAF) {
-            if (state == ST_NEXT) {
-                if (p.prim_mask >= 0) {
-                    k = -1;
-                    if (!single_done) {
-                        single_done = true;
-                        const float *bx = p.boxes + 6 * p.prim_mask;
-                        if (ray_box<R>(o_f, d_f, inv_f, bx, bx + 3, t_min, t_best))
-                            k = p.prim_mask;
-                    }
-                } else {
-                    k = bvh_next<R>(p.bvh, bvh_cur, bvh_from, o_f, d_f, inv_f, t_min, t_best);
-                }
-                if (k < 0) {
-                    state = ST_DONE;
-                    break;
-                }
-                if (STATS)
-                    ++st.tri;
-                T = p.tris + k;
-                // The ray in tangent space (DisplacedIntersector::intersect :178),
-                // formed in double.
-                double rel[3] = {o[0] - T->q0[0], o[1] - T->q0[1], o[2] - T->q0[2]};
-                R omag = 0, dmag = 0;
-                for (int q = 0; q < 3; ++q) {
-                    const double *row = T->T + 3 * q;
-                    wk.o[q] = (R)(row[0] * rel[0] + row[1] * rel[1] + row[2] * rel[2]);
-                    wk.d[q] = (R)(row[0] * d[0] + row[1] * d[1] + row[2] * d[2]);
-                    wk.inv[q] = (R)1 / wk.d[q];
-                    omag = fmax(omag, fabs(wk.o[q]));
-                    dmag = fmax(dmag, fabs(wk.d[q]));
-                }
-                wk.omag = omag;
-                wk.dmag = dmag;
-                wk.si = wk.d[0] < (R)0;
-                wk.sj = wk.d[1] < (R)0;
-                root_level = T->root_level;
-                n_roots = T->n_roots;
-                r = 0;
-                state = ST_ROOT;
-                continue;
+        if (have && state == ST_DONE) {
+            // The result of the finished ray.
+            if (p.any_hit) {
+                p.occluded[rid] = hit_tri >= 0 ? 1 : 0;
+            } else {
+                // The raw record; k_finalize turns it into the hit.
+                const HitsDev &H = p.hits;
+                H.t[rid] = hit_tri >= 0 ? (float)t_best : INFINITY;
+                H.prim[rid] = hit_tri;
+                H.texel_i[rid] = hit_i;
+                H.texel_j[rid] = hit_j;
+                H.s[rid] = (float)hit_a;
+                H.tt[rid] = (float)hit_b;
             }
-            if (state == ST_ROOT) {
-                if (r >= n_roots) {
-                    state = ST_NEXT;
+            if (STATS) {
+                const StatsDev &S = p.stats;
+                if (S.triangle_tests)
+                    S.triangle_tests[rid] = st.tri;
+                if (S.node_tests)
+                    S.node_tests[rid] = st.node;
+                if (S.leaf_tests)
+                    S.leaf_tests[rid] = st.leaf;
+                if (S.sub_squares)
+                    S.sub_squares[rid] = st.sub;
+                if (S.newton_runs)
+                    S.newton_runs[rid] = st.runs;
+                if (S.newton_iters)
+                    S.newton_iters[rid] = st.iters;
+            }
+            have = false;
+        }
+        if (!drained) {
+            uint64_t idle = __ballot(!have);
+            if (__popcll(idle) >= REFILL || idle == ~0ull) {
+                int cnt = __popcll(idle);
+                int leader = __ffsll((unsigned long long)idle) - 1;
+                unsigned base = 0;
+                if (lane == leader)
+                    base = atomicAdd(p.queue, (unsigned)cnt);
+                base = __shfl(base, leader);
+                if ((int64_t)base + cnt >= p.n_rays)
+                    drained = true;
+                if (!have) {
+                    int64_t slot = (int64_t)base + __popcll(idle & ((1ull << lane) - 1));
+                    if (slot < p.n_rays) {
+                        have = true;
+                        rid = p.perm ? (int64_t)p.perm[slot] : slot;
+                        o[0] = p.rays.ox[rid];
+                        o[1] = p.rays.oy[rid];
+                        o[2] = p.rays.oz[rid];
+                        d[0] = p.rays.dx[rid];
+                        d[1] = p.rays.dy[rid];
+                        d[2] = p.rays.dz[rid];
+                        t_min = (R)p.rays.tmin[rid];
+                        t_best = (R)p.rays.tmax[rid];
+                        for (int q = 0; q < 3; ++q) {
+                            o_f[q] = (R)o[q];
+                            d_f[q] = (R)d[q];
+                            inv_f[q] = (R)1 / d_f[q];
+                        }
+                        state = ST_NEED;
+                        walking = false;
+                        bvh_cur = 0;
+                        bvh_from = FROM_PARENT;
+                        single_done = false;
+                        k = -1;
+                        hit_tri = -1;
+                        hit_i = hit_j = 0;
+                        hit_a = hit_b = 0;
+                        st = LaneStats{0, 0, 0, 0, 0, 0};
+                    }
+                }
+            }
+        }
+        if (!__any(have))
+            break;
+        // Bookkeeping, per lane, up to its next block of node tests or leaf.
+        while (state == ST_NEED) {
+            if (walking) {
+                uint32_t m = (uint32_t)(trail >> (4 * lvl)) & 15u;
+                if (m == 0) {
+                    if (lvl >= root_level) {
+                        walking = false;
+                        continue;
+                    }
+                    ++lvl;
+                    pi >>= 1;
+                    pj >>= 1;
                     continue;
                 }
-                int ri = T->root_i[r], rj = T->root_j[r];
-                ++r;
-                R w = ldexp((R)1, root_level) / (R)p.pyr.n_leaf;
-                if (square_outside<R>(*T, ((R)ri + (R)0.5) * w, ((R)rj + (R)0.5) * w, (R)0.5 * w))
-                    continue;
-                if (STATS)
-                    ++st.node;
-                bool slab = p.bound != 0 && root_level <= p.slab_max_level;
-                float4 tp = slab ? fetch_tp(p.pyr, root_level, ri, rj) : make_float4(0.f, 0.f, 0.f, 0.f);
-                if (!cell_test<R, CN>(p, *T, wk, root_level, ri, rj, fetch_mm(p.pyr, root_level, ri, rj), tp, t_min,
-                                      t_best))
-                    continue;
-                if (root_level == 0) {
-                    leaf_i = ri;
-                    leaf_j = rj;
-                    after_leaf = ST_ROOT;
+                int kk = __ffs(m) - 1;
+                trail &= ~((uint64_t)1 << (4 * lvl + kk));
+                int io = (kk >> 1) ^ wk.si, jo = (kk & 1) ^ wk.sj;
+                int ci = lvl == root_level ? rb_i + io : 2 * pi + io;
+                int cj = lvl == root_level ? rb_j + jo : 2 * pj + jo;
+                if (lvl == 0) {
+                    leaf_i = ci;
+                    leaf_j = cj;
                     state = ST_LEAF;
                     break;
                 }
-                int tests = 0;
-                uint32_t mask = expand<R, CN>(p, *T, wk, root_level - 1, ri, rj, t_min, t_best, tests);
-                if (STATS)
-                    st.node += tests;
-                if (mask) {
-                    lvl = root_level - 1;
-                    pi = ri;
-                    pj = rj;
-                    trail = (uint64_t)mask << (4 * lvl);
-                    state = ST_WALK;
-                }
-                continue;
-            }
-            // ST_WALK: the next child of the deepest group with one left.
-            uint32_t m = (uint32_t)(trail >> (4 * lvl)) & 15u;
-            if (m == 0) {
-                if (lvl >= root_level - 1) {
-                    state = ST_ROOT;
-                    continue;
-                }
-                ++lvl;
-                pi >>= 1;
-                pj >>= 1;
-                continue;
-            }
-            int kk = __ffs(m) - 1;
-            trail &= ~((uint64_t)1 << (4 * lvl + kk));
-            int ci = 2 * pi + ((kk >> 1) ^ wk.si), cj = 2 * pj + ((kk & 1) ^ wk.sj);
-            if (lvl == 0) {
-                if (STATS)
-                    ++st.leaf;
-                leaf_i = ci;
-                leaf_j = cj;
-                after_leaf = ST_WALK;
-                state = ST_LEAF;
+                ex_level = lvl - 1;
+                ex_i = 2 * ci;
+                ex_j = 2 * cj;
+                ex_aligned = true;
+                ex_valid = 15;
+                state = ST_EXPAND;
                 break;
             }
+            // The next triangle whose box the ray meets (intersect_flat,
+            // displaced_intersector.cpp:578-599, through the BVH).
+            if (p.prim_mask >= 0) {
+                k = -1;
+                if (!single_done) {
+                    single_done = true;
+                    const float *bx = p.boxes + 6 * p.prim_mask;
+                    if (ray_box<R>(o_f, d_f, inv_f, bx, bx + 3, t_min, t_best))
+                        k = p.prim_mask;
+                }
+            } else {
+                k = bvh_next<R>(p.bvh, bvh_cur, bvh_from, o_f, d_f, inv_f, t_min, t_best);
+            }
+            if (k < 0) {
+                state = ST_DONE;
+                break;
+            }
+            if (STATS)
+                ++st.tri;
+            T = p.tris + k;
+            // The ray in tangent space (DisplacedIntersector::intersect :178),
+            // formed in double.
+            double rel[3] = {o[0] - T->q0[0], o[1] - T->q0[1], o[2] - T->q0[2]};
+            R omag = 0, dmag = 0;
+            for (int q = 0; q < 3; ++q) {
+                const double *row = T->T + 3 * q;
+                wk.o[q] = (R)(row[0] * rel[0] + row[1] * rel[1] + row[2] * rel[2]);
+                wk.d[q] = (R)(row[0] * d[0] + row[1] * d[1] + row[2] * d[2]);
+                wk.inv[q] = (R)1 / wk.d[q];
+                omag = fmax(omag, fabs(wk.o[q]));
+                dmag = fmax(dmag, fabs(wk.d[q]));
+            }
+            wk.omag = omag;
+            wk.dmag = dmag;
+            wk.si = wk.d[0] < (R)0;
+            wk.sj = wk.d[1] < (R)0;
+            root_level = T->root_level;
+            // The roots: the cells of the footprint's 2 x 2 root block that
+            // meet the domain's bounding box (displaced_intersector.cpp:139-149).
+            rb_i = T->root_i[0];
+            rb_j = T->root_j[0];
+            uint32_t rv = 0;
+            for (int r = 0; r < T->n_roots; ++r)
+                rv |= 1u << ((T->root_i[r] - rb_i) + 2 * (T->root_j[r] - rb_j));
+            ex_level = root_level;
+            ex_i = rb_i;
+            ex_j = rb_j;
+            ex_aligned = false;
+            ex_valid = rv;
+            state = ST_EXPAND;
+        }
+        if (state == ST_EXPAND) {
             int tests = 0;
-            uint32_t mask = expand<R, CN>(p, *T, wk, lvl - 1, ci, cj, t_min, t_best, tests);
+            uint32_t mask = expand<R, CN>(p, *T, wk, ex_level, ex_i, ex_j, ex_aligned, ex_valid, t_min, t_best, tests);
             if (STATS)
                 st.node += tests;
-            if (mask) {
-                --lvl;
-                pi = ci;
-                pj = cj;
+            if (!ex_aligned) {
+                // the root block
+                lvl = root_level;
+                trail = (uint64_t)mask << (4 * lvl);
+                walking = true;
+            } else if (mask) {
+                lvl = ex_level;
+                pi = ex_i >> 1;
+                pj = ex_j >> 1;
                 trail |= (uint64_t)mask << (4 * lvl);
             }
+            state = ST_NEED;
         }
-        if (__all(state == ST_DONE))
-            break;
-        if (state == ST_LEAF) {
-            if (STATS && after_leaf == ST_ROOT)
-                ++st.leaf;
-            Leaf<R> L;
-            leaf_setup<R>(p, *T, o, d, leaf_i, leaf_j, L);
-            R la = 0, lb = 0;
-            bool found;
-            if (p.leaf == 0)
-                found = search_leaf<R, CN>(*T, L, t_min, t_best, la, lb, s_t, s_code, st);
-            else
-                found = leaf_two_triangle<R>(p, *T, L, o, d, t_min, t_best, la, lb);
-            if (found) {
-                hit_tri = k;
-                hit_i = leaf_i;
-                hit_j = leaf_j;
-                hit_a = la;
-                hit_b = lb;
+        uint64_t leaf_lanes = __ballot(state == ST_LEAF);
+        uint64_t busy_lanes = __ballot(state != ST_DONE);
+        if (leaf_lanes && (leaf_lanes == busy_lanes ||
+                           64 * __popcll(leaf_lanes) >= LEAF_BATCH * __popcll(busy_lanes))) {
+            if (state == ST_LEAF) {
+                if (STATS)
+                    ++st.leaf;
+                Leaf<R> L;
+                leaf_setup<R>(p, *T, o, d, leaf_i, leaf_j, L);
+                R la = 0, lb = 0;
+                bool found;
+                if (p.leaf == 0)
+                    found = search_leaf<R, CN>(*T, L, t_min, t_best, la, lb, s_t, s_code, st);
+                else
+                    found = leaf_two_triangle<R>(p, *T, L, o, d, t_min, t_best, la, lb);
+                if (found) {
+                    hit_tri = k;
+                    hit_i = leaf_i;
+                    hit_j = leaf_j;
+                    hit_a = la;
+                    hit_b = lb;
+                }
+                state = (found && p.any_hit) ? ST_DONE : ST_NEED;
             }
-            state = (found && p.any_hit) ? ST_DONE : after_leaf;
         }
     }
 
-    if (!valid)
-        return;
-    if (p.any_hit) {
-        p.occluded[rid] = hit_tri >= 0 ? 1 : 0;
-    } else {
-        // The raw record; k_finalize turns it into the hit.
-        const HitsDev &H = p.hits;
-        H.t[rid] = hit_tri >= 0 ? (float)t_best : INFINITY;
-        H.prim[rid] = hit_tri;
-        H.texel_i[rid] = hit_i;
-        H.texel_j[rid] = hit_j;
-        H.s[rid] = (float)hit_a;
-        H.tt[rid] = (float)hit_b;
-    }
-    if (STATS) {
-        const StatsDev &S = p.stats;
-        if (S.triangle_tests)
-            S.triangle_tests[rid] = st.tri;
-        if (S.node_tests)
-            S.node_tests[rid] = st.node;
-        if (S.leaf_tests)
-            S.leaf_tests[rid] = st.leaf;
-        if (S.sub_squares)
-            S.sub_squares[rid] = st.sub;
-        if (S.newton_runs)
-            S.newton_runs[rid] = st.runs;
-        if (S.newton_iters)
-            S.newton_iters[rid] = st.iters;
-    }
 }
 
 // The reported hit, in double: the root polished (Newton leaf) or the face
@@ -1440,6 +1500,8 @@ hipError_t launch_height_range(const PyramidView &P, const TriRecord *tris, int 
     return hipGetLastError();
 }
 
+__global__ void k_queue_reset(unsigned *queue) { *queue = 0u; }
+
 template <class R, bool STATS>
 static void launch_trace_r(const TraceParams &p, unsigned blocks, hipStream_t stream)
 {
@@ -1453,7 +1515,18 @@ hipError_t launch_trace(const TraceParams &p, int precision, hipStream_t stream)
 {
     if (p.n_rays <= 0)
         return hipSuccess;
-    unsigned blocks = (unsigned)((p.n_rays + BLOCK - 1) / BLOCK);
+    // Persistent waves: enough workgroups to fill every CU a few times over.
+    static int cus = 0;
+    if (cus == 0) {
+        int dev = 0;
+        hipGetDevice(&dev);
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+        if (cus <= 0)
+            cus = 256;
+    }
+    int64_t want = (p.n_rays + BLOCK - 1) / BLOCK;
+    unsigned blocks = (unsigned)(want < (int64_t)cus * 4 ? want : (int64_t)cus * 4);
+    hipLaunchKernelGGL(k_queue_reset, dim3(1), dim3(1), 0, stream, p.queue);
     bool stats = p.stats.node_tests || p.stats.leaf_tests || p.stats.sub_squares || p.stats.triangle_tests ||
                  p.stats.newton_runs || p.stats.newton_iters;
     if (precision == 1) {

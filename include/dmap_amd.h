@@ -58,6 +58,8 @@ typedef struct
     int slab_max_level; /* the slab is read at levels <= this (default 1) */
     int leaf;           /* 0 certified Newton (default), 1 two-triangle */
     int precision;      /* 0 fp32 (default), 1 the same solve in fp64 on the device */
+    int reorder;        /* 1 (default): sort the rays by direction octant and origin before
+                           tracing, for wave coherence; results are per ray either way */
 } dmap_options;
 
 /* Device pointers, n entries each. */
