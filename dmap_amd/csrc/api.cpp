@@ -823,6 +823,8 @@ static int trace(const dmap_object *obj, const dmap_rays *rays, int64_t n, const
 int dmap_intersect_closest(const dmap_object *obj, const dmap_rays *rays, int64_t n, const dmap_hits *hits,
                            int prim_mask, const dmap_options *options, const dmap_ray_stats *stats, void *stream)
 {
+    if (n == 0 && obj && rays)
+        return DMAP_OK;
     if (!hits || !hits->t || !hits->s || !hits->tt || !hits->prim || !hits->texel_i || !hits->texel_j)
         return fail(DMAP_ERR_INVALID, "dmap_intersect_closest: hits t, s, tt, prim, texel_i and texel_j are required");
     return trace(obj, rays, n, hits, nullptr, prim_mask, options, stats, stream);
@@ -831,6 +833,8 @@ int dmap_intersect_closest(const dmap_object *obj, const dmap_rays *rays, int64_
 int dmap_intersect_any(const dmap_object *obj, const dmap_rays *rays, int64_t n, uint8_t *occluded, int prim_mask,
                        const dmap_options *options, void *stream)
 {
+    if (n == 0 && obj && rays)
+        return DMAP_OK;
     if (!occluded)
         return fail(DMAP_ERR_INVALID, "dmap_intersect_any: occluded is NULL");
     return trace(obj, rays, n, nullptr, occluded, prim_mask, options, nullptr, stream);

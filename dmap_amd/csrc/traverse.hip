@@ -1178,7 +1178,13 @@ __global__ void __launch_bounds__(BLOCK) k_trace(TraceParams p)
                             d_f[q] = (R)d[q];
                             inv_f[q] = (R)1 / d_f[q];
                         }
-                        state = ST_NEED;
+                        // A ray with a non-finite or zero component, or an empty
+                        // (t_min, t_max), has no hit: the reference's walk finds none.
+                        bool ok = isfinite(o_f[0]) && isfinite(o_f[1]) && isfinite(o_f[2]) && isfinite(d_f[0]) &&
+                                  isfinite(d_f[1]) && isfinite(d_f[2]) &&
+                                  (d_f[0] != (R)0 || d_f[1] != (R)0 || d_f[2] != (R)0) && t_min < t_best &&
+                                  !isnan(t_best);
+                        state = ok ? ST_NEED : ST_DONE;
                         walking = false;
                         bvh_cur = 0;
                         bvh_from = FROM_PARENT;

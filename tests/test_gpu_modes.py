@@ -1,0 +1,137 @@
+"""GPU against the oracle across the traversal options and object kinds:
+node-test modes, the two-triangle leaf, any-hit, prim_mask, the fp64
+device run, interpolated normals, general charts, clamp and offset."""
+import numpy as np
+import pytest
+
+from dmap_amd import workload as W
+from tests import scenes
+from tests.gpu_util import gpu_hits, oracle_hits
+from tests.parity import compare
+
+pytestmark = pytest.mark.gpu
+
+
+def build(tex, strength, offset, repeat, tris, build_mode="fold"):
+    import dmap_amd
+    tile = dmap_amd.DisplacementTile(tex, strength, offset, repeat, build_mode)
+    return dmap_amd.ObjectIntersector(tile, tris)
+
+
+def summary(c):
+    return {k: v for k, v in c.items() if k != "bad_index"}
+
+
+@pytest.mark.parametrize("bound,k", [("box", 1), ("slab", 1), ("both", 0), ("both", 3), ("slab", 8)])
+def test_bound_modes(torch_gpu, bound, k):
+    import dmap_amd
+    tex = np.random.default_rng(4).random((128, 128)).astype(np.float32)
+    tris = W.quad_plane(1, 1.0)
+    obj = build(tex, 0.06, 0.0, True, tris)
+    rays = scenes.rays(40000, 5)
+    opt = dmap_amd.TraversalOptions(bound=bound, slab_max_level=k)
+    g = gpu_hits(obj, rays, options=opt)
+    ref, d, _ = oracle_hits(W.close_tile(tex), 0.06, 0.0, True, tris, rays, bound=bound, slab_max_level=k)
+    c = compare(g, ref, d, tris, 128)
+    print(summary(c))
+    assert c["unexplained"] == 0, summary(c)
+
+
+@pytest.mark.parametrize("kind", ["curved", "rotated"])
+def test_general_normals_and_charts(torch_gpu, kind):
+    tex = np.random.default_rng(6).random((64, 64)).astype(np.float32)
+    tris = scenes.curved_quad() if kind == "curved" else scenes.rotated_chart_quad()
+    obj = build(tex, 0.05, 0.0, True, tris)
+    rays = scenes.rays(40000, 7)
+    g = gpu_hits(obj, rays)
+    ref, d, _ = oracle_hits(W.close_tile(tex), 0.05, 0.0, True, tris, rays)
+    c = compare(g, ref, d, tris, 64)
+    print(summary(c))
+    assert c["hits_ref"] > 10000
+    assert c["unexplained"] == 0, summary(c)
+    both = np.isfinite(g["t"]) & np.isfinite(ref["t"])
+    assert np.abs(g["bary"][both] - ref["bary"][both]).max() < 1e-4
+
+
+def test_clamp_and_offset_and_direct_build(torch_gpu):
+    tex = np.random.default_rng(8).random((64, 64)).astype(np.float32)
+    tris = W.quad_plane(1, 1.0)
+    obj = build(tex, 0.07, -0.035, False, tris, "direct")
+    rays = scenes.rays(30000, 9)
+    g = gpu_hits(obj, rays)
+    ref, d, _ = oracle_hits(W.close_tile(tex, repeat=False), 0.07, -0.035, False, tris, rays, build="direct")
+    c = compare(g, ref, d, tris, 64)
+    print(summary(c))
+    assert c["unexplained"] == 0, summary(c)
+
+
+def test_two_triangle_leaf(torch_gpu):
+    import dmap_amd
+    tex = np.random.default_rng(10).random((64, 64)).astype(np.float32)
+    tris = W.quad_plane(1, 1.0)
+    obj = build(tex, 0.05, 0.0, True, tris)
+    rays = scenes.rays(40000, 11)
+    g = gpu_hits(obj, rays, options=dmap_amd.TraversalOptions(leaf="two_triangle"))
+    ref, d, _ = oracle_hits(W.close_tile(tex), 0.05, 0.0, True, tris, rays, leaf="two_triangle")
+    # The texel mesh's own edges (texel boundaries and the (i+1, j)-(i, j+1)
+    # diagonals) are where fp32 and fp64 crossings may fall on either side.
+    st = np.where(np.isfinite(ref["t"])[:, None], ref["st"], g["st"])
+    f = st * 64 - np.floor(st * 64)
+    near = (np.minimum(f, 1 - f) < 1e-4).any(axis=1) | (np.abs(f.sum(axis=1) - 1) < 1e-4)
+    c = compare(g, ref, d, tris, 64)
+    bad = c["bad_index"]
+    print(summary(c), "on mesh edges:", int(near[bad].sum()))
+    assert (~near[bad]).sum() == 0
+
+
+def test_any_hit_and_prim_mask(torch_gpu):
+    import dmap_amd
+    import torch
+    tex = W.c2_map(256)
+    tris = W.quad_plane(8, 8.0)
+    obj = build(tex, 0.1, 0.0, True, tris)
+    wl = W.c2_incoherent(n=256, width=128, height=64, n_bounce=8192)
+    rays = wl.rays
+    g = gpu_hits(obj, rays)
+    occ = obj.occluded(torch.from_numpy(rays).cuda()).cpu().numpy().astype(bool)
+    assert (occ == np.isfinite(g["t"])).all()
+    short = rays.copy()
+    short[7] = np.where(np.isfinite(g["t"]), g["t"] * 0.999, np.inf)
+    occ2 = obj.occluded(torch.from_numpy(short).cuda()).cpu().numpy().astype(bool)
+    assert not occ2[np.isfinite(g["t"])].any()
+    # prim_mask = k: the walk of triangle k alone (the emitter-pdf door).
+    for k in (0, 37, 127):
+        gk = gpu_hits(obj, rays, prim_mask=k)
+        ref, d, _ = oracle_hits(W.close_tile(tex), 0.1, 0.0, True, tris[k:k + 1], rays)
+        ref["prim"] = np.where(ref["prim"] >= 0, k, -1)
+        c = compare(gk, ref, d, tris, 256)
+        assert c["unexplained"] == 0, summary(c)
+        assert ((gk["prim"] == k) | (gk["prim"] == -1)).all()
+
+
+def test_fp64_device_run(torch_gpu):
+    """The same solve in fp64 on the device: agrees with the double oracle
+    to the Newton tolerance, and gives the fp32 error report."""
+    import dmap_amd
+    wl = W.c2_incoherent(n=256, width=128, height=128, n_bounce=16384)
+    obj = build(wl.texels, 0.1, 0.0, True, wl.triangles)
+    g64 = gpu_hits(obj, wl.rays, options=dmap_amd.TraversalOptions(precision="fp64"))
+    g32 = gpu_hits(obj, wl.rays)
+    ref, d, _ = oracle_hits(wl.nodes, 0.1, 0.0, True, wl.triangles, wl.rays)
+    c = compare(g64, ref, d, wl.triangles, 256, rel=1e-6)
+    print("fp64", summary(c))
+    assert c["unexplained"] == 0 and c["hitmiss_flips"] == 0
+    both = np.isfinite(g32["t"]) & np.isfinite(g64["t"])
+    err = np.abs(g32["t"][both] - g64["t"][both]) / g64["t"][both]
+    print(f"fp32 vs fp64 t: max {err.max():.2e}, p99 {np.quantile(err, 0.99):.2e}")
+    assert (np.isfinite(g32["t"]) == np.isfinite(g64["t"])).mean() > 0.9999
+
+
+def test_reorder_does_not_change_results(torch_gpu):
+    import dmap_amd
+    wl = W.c2_incoherent(n=256, width=128, height=128, n_bounce=16384)
+    obj = build(wl.texels, 0.1, 0.0, True, wl.triangles)
+    a = gpu_hits(obj, wl.rays, options=dmap_amd.TraversalOptions(reorder=True))
+    b = gpu_hits(obj, wl.rays, options=dmap_amd.TraversalOptions(reorder=False))
+    for key in ("t", "prim", "st", "ng", "texel"):
+        assert np.array_equal(a[key], b[key]), key
