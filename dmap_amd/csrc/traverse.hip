@@ -1083,8 +1083,13 @@ constexpr int REFILL = 16;
 // from the BVH) to its next block of node tests, then runs those node
 // tests for the whole wave at once; pending leaves are searched together
 // once enough lanes have one.
-template <class R, bool STATS, bool CN>
-__global__ void __launch_bounds__(BLOCK) k_trace(TraceParams p)
+#ifndef DMAP_TRACE_WAVES
+#define DMAP_TRACE_WAVES 3
+#endif
+
+template <class R, bool STATS, bool CN, bool TWO>
+__global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(DMAP_TRACE_WAVES, 8)))
+k_trace(TraceParams p)
 {
     __shared__ typename StackSlot<R>::T s_t[STACK * BLOCK];
     __shared__ uint16_t s_code[STACK * BLOCK];
@@ -1096,9 +1101,9 @@ __global__ void __launch_bounds__(BLOCK) k_trace(TraceParams p)
     // drained and its lanes are done.
     bool drained = false, have = false;
     int64_t rid = 0;
-    double o[3] = {0, 0, 0}, d[3] = {0, 0, 1};
     R t_min = 0, t_best = 0;
-    R o_f[3], d_f[3], inv_f[3];
+    // The ray as given (fp32, so exact in R and in double).
+    R o_f[3] = {0, 0, 0}, d_f[3] = {0, 0, 1}, inv_f[3] = {0, 0, 0};
     int state = ST_DONE;
     bool walking = false;
     int bvh_cur = 0, bvh_from = FROM_PARENT;
@@ -1165,19 +1170,16 @@ __global__ void __launch_bounds__(BLOCK) k_trace(TraceParams p)
                     if (slot < p.n_rays) {
                         have = true;
                         rid = p.perm ? (int64_t)p.perm[slot] : slot;
-                        o[0] = p.rays.ox[rid];
-                        o[1] = p.rays.oy[rid];
-                        o[2] = p.rays.oz[rid];
-                        d[0] = p.rays.dx[rid];
-                        d[1] = p.rays.dy[rid];
-                        d[2] = p.rays.dz[rid];
+                        o_f[0] = p.rays.ox[rid];
+                        o_f[1] = p.rays.oy[rid];
+                        o_f[2] = p.rays.oz[rid];
+                        d_f[0] = p.rays.dx[rid];
+                        d_f[1] = p.rays.dy[rid];
+                        d_f[2] = p.rays.dz[rid];
                         t_min = (R)p.rays.tmin[rid];
                         t_best = (R)p.rays.tmax[rid];
-                        for (int q = 0; q < 3; ++q) {
-                            o_f[q] = (R)o[q];
-                            d_f[q] = (R)d[q];
+                        for (int q = 0; q < 3; ++q)
                             inv_f[q] = (R)1 / d_f[q];
-                        }
                         // A ray with a non-finite or zero component, or an empty
                         // (t_min, t_max), has no hit: the reference's walk finds none.
                         bool ok = isfinite(o_f[0]) && isfinite(o_f[1]) && isfinite(o_f[2]) && isfinite(d_f[0]) &&
@@ -1255,12 +1257,12 @@ __global__ void __launch_bounds__(BLOCK) k_trace(TraceParams p)
             T = p.tris + k;
             // The ray in tangent space (DisplacedIntersector::intersect :178),
             // formed in double.
-            double rel[3] = {o[0] - T->q0[0], o[1] - T->q0[1], o[2] - T->q0[2]};
+            double rel[3] = {(double)o_f[0] - T->q0[0], (double)o_f[1] - T->q0[1], (double)o_f[2] - T->q0[2]};
             R omag = 0, dmag = 0;
             for (int q = 0; q < 3; ++q) {
                 const double *row = T->T + 3 * q;
                 wk.o[q] = (R)(row[0] * rel[0] + row[1] * rel[1] + row[2] * rel[2]);
-                wk.d[q] = (R)(row[0] * d[0] + row[1] * d[1] + row[2] * d[2]);
+                wk.d[q] = (R)(row[0] * (double)d_f[0] + row[1] * (double)d_f[1] + row[2] * (double)d_f[2]);
                 wk.inv[q] = (R)1 / wk.d[q];
                 omag = fmax(omag, fabs(wk.o[q]));
                 dmag = fmax(dmag, fabs(wk.d[q]));
@@ -1309,11 +1311,13 @@ __global__ void __launch_bounds__(BLOCK) k_trace(TraceParams p)
             if (state == ST_LEAF) {
                 if (STATS)
                     ++st.leaf;
+                const double o[3] = {(double)o_f[0], (double)o_f[1], (double)o_f[2]};
+                const double d[3] = {(double)d_f[0], (double)d_f[1], (double)d_f[2]};
                 Leaf<R> L;
                 leaf_setup<R>(p, *T, o, d, leaf_i, leaf_j, L);
                 R la = 0, lb = 0;
                 bool found;
-                if (p.leaf == 0)
+                if (!TWO)
                     found = search_leaf<R, CN>(*T, L, t_min, t_best, la, lb, s_t, s_code, st);
                 else
                     found = leaf_two_triangle<R>(p, *T, L, o, d, t_min, t_best, la, lb);
@@ -1508,13 +1512,22 @@ hipError_t launch_height_range(const PyramidView &P, const TriRecord *tris, int 
 
 __global__ void k_queue_reset(unsigned *queue) { *queue = 0u; }
 
+template <class R, bool STATS, bool TWO>
+static void launch_trace_l(const TraceParams &p, unsigned blocks, hipStream_t stream)
+{
+    if (p.const_normal)
+        hipLaunchKernelGGL((k_trace<R, STATS, true, TWO>), dim3(blocks), dim3(BLOCK), 0, stream, p);
+    else
+        hipLaunchKernelGGL((k_trace<R, STATS, false, TWO>), dim3(blocks), dim3(BLOCK), 0, stream, p);
+}
+
 template <class R, bool STATS>
 static void launch_trace_r(const TraceParams &p, unsigned blocks, hipStream_t stream)
 {
-    if (p.const_normal)
-        hipLaunchKernelGGL((k_trace<R, STATS, true>), dim3(blocks), dim3(BLOCK), 0, stream, p);
+    if (p.leaf == 1)
+        launch_trace_l<R, STATS, true>(p, blocks, stream);
     else
-        hipLaunchKernelGGL((k_trace<R, STATS, false>), dim3(blocks), dim3(BLOCK), 0, stream, p);
+        launch_trace_l<R, STATS, false>(p, blocks, stream);
 }
 
 hipError_t launch_trace(const TraceParams &p, int precision, hipStream_t stream)
