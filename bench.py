@@ -50,6 +50,12 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=4, help="oracle sample: every k-th ray")
     ap.add_argument("--precision", default="fp32", choices=["fp32", "fp64"])
     ap.add_argument("--no-reorder", action="store_true", help="trace the rays in their given order")
+    ap.add_argument("--serve-seconds", type=float, default=2.0,
+                    help="length of the fixed-rate serving run (0: no serving section)")
+    ap.add_argument("--serve-load", type=float, default=0.8,
+                    help="serving arrival rate as a fraction of the plain kernel's Mrays/s on the same rays")
+    ap.add_argument("--serve-slots", type=int, default=4)
+    ap.add_argument("--serve-step", type=int, default=1 << 18, help="rays per serving step")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus}); "
@@ -153,6 +159,9 @@ def main():
                "gpu_over_cpu": round(value / cpu_mrays, 1)}
         log(f"[bench] cpu {cpu_mrays:.3f} Mrays/s; gpu/cpu {value / cpu_mrays:.1f}x")
 
+    if args.serve_seconds > 0:
+        extra["serving"] = serve(args, log)
+
     out = {"metric": METRIC, "value": round(value, 2), "unit": "Mrays/s", "n_gpus": 1, "steps": args.steps,
            "warmup": args.warmup, "ms_per_step": round(ms_per_step, 4), "higher_is_better": True,
            "scaling": "weak", "vs_baseline": None, "dtype": args.precision, "data": "synthetic",
@@ -165,6 +174,56 @@ def main():
                              "GB_s": round(build_bytes / (build_ms * 1e-3) / 1e9, 1)},
            **extra}
     print(json.dumps(out), flush=True)
+
+
+def serve(args, log):
+    """Continuous batching over 8 resident scenes: the plain kernel's rate on
+    the same rays, the server's saturated rate, then Poisson arrivals at a
+    fixed fraction of the plain rate with p50/p99 latency."""
+    import torch
+
+    import dmap_amd
+    from dmap_amd import serving as SV
+
+    t0 = time.time()
+    scenes = SV.make_scenes()
+    reqs = SV.make_requests(scenes, 256, seed=17)
+    mean_rays = float(np.mean([r.shape[1] for _, r in reqs]))
+    log(f"[serve] 8 scenes {SV.SCENE_SIZES}, 256 distinct requests (mean {mean_rays:.0f} rays) "
+        f"in {time.time() - t0:.1f}s")
+    plain = SV.plain_rate(scenes, reqs)
+    log(f"[serve] plain batched kernel on the same rays: {plain:.1f} Mrays/s")
+    srv = dmap_amd.Server(slots=args.serve_slots, step_rays=args.serve_step, hit_pool_bytes=8 << 30)
+    # Accuracy: the first 16 requests through the server against the
+    # single-request entry point.
+    same = 0
+    for q in range(16):
+        k, rays = reqs[q]
+        srv.submit(scenes[k].obj, rays, q)
+    got = 0
+    while got < 16:
+        for tag, _ in srv.poll():
+            k, rays = reqs[tag]
+            a = srv.hits(tag, rays.shape[1])
+            b = scenes[k].obj.intersect(rays)
+            same += int(all(torch.equal(a[f], b[f]) for f in ("t", "st", "bary", "ng", "prim", "texel")))
+            srv.release(tag)
+            got += 1
+    sat = SV.summarize(SV.drive(srv, scenes, reqs, None, 0, seed=1))
+    log(f"[serve] saturated: {sat}")
+    rate = args.serve_load * plain * 1e6 / mean_rays
+    fixed = SV.summarize(SV.drive(srv, scenes, reqs, rate, args.serve_seconds, seed=2, log=log))
+    log(f"[serve] fixed rate {rate:.0f} req/s: {fixed}")
+    del srv
+    return {"workload": "8 resident scenes (512^2, 1024^2, 2048^2 x3, 4096^2 x2, 8192^2 fBm+ridged, 64 quads each), "
+                        "requests log-uniform in [1k, 1M] rays, 70% coherent primaries / 30% bounces, "
+                        "Poisson arrivals",
+            "slots": args.serve_slots, "step_rays": args.serve_step,
+            "plain_mrays_s": round(plain, 2),
+            "saturated": sat, "saturated_over_plain": round(sat["sustained_mrays_s"] / plain, 3),
+            "fixed_rate": {"arrivals_per_s": round(rate, 1), "load_vs_plain": args.serve_load, **fixed},
+            "fixed_over_plain": round(fixed["sustained_mrays_s"] / plain, 3),
+            "identical_to_single_request": f"{same}/16"}
 
 
 if __name__ == "__main__":

@@ -35,7 +35,8 @@ STAT_NAMES = ("triangle_tests", "node_tests", "leaf_tests", "sub_squares", "newt
 EXPORTS = ("dmap_last_error", "dmap_version", "dmap_ctx_create", "dmap_ctx_destroy", "dmap_tile_create",
            "dmap_tile_rebuild", "dmap_tile_info", "dmap_tile_read_level", "dmap_tile_read_nodes",
            "dmap_tile_destroy", "dmap_object_create", "dmap_object_triangle_info", "dmap_object_destroy",
-           "dmap_intersect_closest", "dmap_intersect_any")
+           "dmap_intersect_closest", "dmap_intersect_any", "dmap_server_create", "dmap_server_submit",
+           "dmap_server_poll", "dmap_server_hits", "dmap_server_copy_hits", "dmap_server_release", "dmap_server_idle", "dmap_server_destroy")
 
 
 class DmapError(RuntimeError):
@@ -280,3 +281,73 @@ class ObjectIntersector:
         if getattr(self, "_h", None) and self._h.value and _lib is not None:
             _lib.dmap_object_destroy(self._h)
             self._h = None
+
+
+class Server:
+    """Continuous-batching closest-hit server over resident objects
+    (dmap_server_*): submit requests, poll completions with latencies,
+    read and release their hits."""
+
+    def __init__(self, slots=4, step_rays=1 << 18, hit_pool_bytes=4 << 30, ctx=None):
+        L = load_library()
+        c = ctypes
+        vp = c.c_void_p
+        L.dmap_server_create.argtypes = [vp, c.c_int, c.c_int64, c.c_int64, c.POINTER(vp)]
+        L.dmap_server_submit.argtypes = [vp, vp, c.POINTER(_Rays), c.c_int64, c.c_uint64]
+        L.dmap_server_poll.argtypes = [vp, c.c_int, vp, vp, c.POINTER(c.c_int)]
+        L.dmap_server_hits.argtypes = [vp, c.c_uint64, c.POINTER(_Hits)]
+        L.dmap_server_copy_hits.argtypes = [vp, c.c_uint64, c.POINTER(_Hits), vp]
+        L.dmap_server_release.argtypes = [vp, c.c_uint64]
+        L.dmap_server_idle.argtypes = [vp, c.POINTER(c.c_int)]
+        L.dmap_server_destroy.argtypes = [vp]
+        self.ctx = ctx or default_context()
+        self._h = ctypes.c_void_p()
+        _check(L.dmap_server_create(self.ctx._h, int(slots), int(step_rays), int(hit_pool_bytes),
+                                    ctypes.byref(self._h)), "dmap_server_create")
+        self._keep = {}
+        self._tags = np.zeros(4096, dtype=np.uint64)
+        self._lat = np.zeros(4096, dtype=np.float64)
+
+    def submit(self, obj, rays, tag):
+        """rays: (8, n) float32 device tensor, kept alive until released."""
+        r, n = ObjectIntersector._rays(rays)
+        self._keep[int(tag)] = rays
+        _check(_lib.dmap_server_submit(self._h, obj._h, ctypes.byref(r), n, ctypes.c_uint64(int(tag))),
+               "dmap_server_submit")
+
+    def poll(self):
+        """[(tag, latency_s)] of requests completed since the last poll."""
+        out = []
+        while True:
+            k = ctypes.c_int()
+            _check(_lib.dmap_server_poll(self._h, len(self._tags), ctypes.c_void_p(self._tags.ctypes.data),
+                                         ctypes.c_void_p(self._lat.ctypes.data), ctypes.byref(k)), "dmap_server_poll")
+            out += [(int(self._tags[q]), float(self._lat[q])) for q in range(k.value)]
+            if k.value < len(self._tags):
+                return out
+
+    def hits(self, tag, n, stream=None):
+        """A completed request's hits, copied into fresh device tensors in
+        the layout ObjectIntersector.intersect returns."""
+        torch = _torch()
+        F = torch.empty((8, n), dtype=torch.float32, device="cuda")
+        I = torch.empty((3, n), dtype=torch.int32, device="cuda")
+        h = _Hits(*[F[q].data_ptr() for q in range(8)], *[I[q].data_ptr() for q in range(3)])
+        _check(_lib.dmap_server_copy_hits(self._h, ctypes.c_uint64(int(tag)), ctypes.byref(h), _stream_ptr(stream)),
+               "dmap_server_copy_hits")
+        return {"t": F[0], "st": F[1:3], "bary": F[3:5], "ng": F[5:8], "prim": I[0], "texel": I[1:3]}
+
+    def release(self, tag):
+        _check(_lib.dmap_server_release(self._h, ctypes.c_uint64(int(tag))), "dmap_server_release")
+        self._keep.pop(int(tag), None)
+
+    def idle(self):
+        k = ctypes.c_int()
+        _check(_lib.dmap_server_idle(self._h, ctypes.byref(k)), "dmap_server_idle")
+        return bool(k.value)
+
+    def __del__(self):
+        if getattr(self, "_h", None) and self._h.value and _lib is not None:
+            _lib.dmap_server_destroy(self._h)
+            self._h = None
+

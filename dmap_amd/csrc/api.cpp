@@ -14,53 +14,14 @@
 #include <string>
 #include <vector>
 
-#include "../../include/dmap_amd.h"
-#include "kernels.h"
-#include "layout.h"
-
-using namespace dmap_amd;
-
-struct dmap_ctx
-{
-    int device = 0;
-};
-
-struct dmap_tile
-{
-    dmap_ctx *ctx = nullptr;
-    int n = 0, n_levels = 0, repeat = 1, build = 0;
-    double strength = 1.0, offset = 0.0;
-    float *texels = nullptr, *nodes = nullptr;
-    float2 *mm = nullptr;
-    float4 *tp = nullptr;
-    float2 *rr = nullptr;
-    double *scratch = nullptr;
-    size_t cells = 0;
-    float root_hmin = 0.f, root_hmax = 0.f;
-    int64_t bytes = 0;
-};
-
-struct dmap_object
-{
-    dmap_ctx *ctx = nullptr;
-    dmap_tile *tile = nullptr;
-    int n_tri = 0;
-    int const_normal = 0;
-    float bound_lo[3] = {0, 0, 0}, bound_hi[3] = {0, 0, 0};
-    std::vector<TriRecord> host;
-    std::vector<float> boxes_host, h_range_host;
-    std::vector<BvhNode> bvh_host;
-    TriRecord *tris = nullptr;
-    float *boxes = nullptr;
-    BvhNode *bvh = nullptr;
-};
+#include "internal.h"
 
 namespace
 {
-
 thread_local std::string g_error;
+}
 
-int fail(int code, const char *fmt, ...)
+int dmap_amd_internal::fail(int code, const char *fmt, ...)
 {
     char buf[512];
     va_list ap;
@@ -71,13 +32,8 @@ int fail(int code, const char *fmt, ...)
     return code;
 }
 
-#define HIP_TRY(call)                                                                                                 \
-    do {                                                                                                              \
-        hipError_t e_ = (call);                                                                                       \
-        if (e_ != hipSuccess)                                                                                         \
-            return fail(e_ == hipErrorOutOfMemory ? DMAP_ERR_MEMORY : DMAP_ERR_HIP, "%s: %s", #call,                  \
-                        hipGetErrorString(e_));                                                                       \
-    } while (0)
+namespace
+{
 
 // ------------------------------------------------------------ host math
 
@@ -625,6 +581,7 @@ static void object_free(dmap_object *o)
     hipFree(o->tris);
     hipFree(o->boxes);
     hipFree(o->bvh);
+    hipFree(o->view_dev);
     delete o;
 }
 
@@ -697,7 +654,17 @@ int dmap_object_create(dmap_ctx *ctx, dmap_tile *tile, int n_tri, const double *
         o->bound_lo[a] = o->bvh_host[0].lo[a];
         o->bound_hi[a] = o->bvh_host[0].hi[a];
     }
+    o->view.pyr = view_of(tile);
+    o->view.tris = o->tris;
+    o->view.boxes = o->boxes;
+    o->view.n_tri = n_tri;
+    o->view.const_normal = o->const_normal;
     e = hipMalloc(&o->bvh, sizeof(BvhNode) * o->bvh_host.size());
+    o->view.bvh = o->bvh;
+    if (e == hipSuccess)
+        e = hipMalloc(&o->view_dev, sizeof(ObjectDev));
+    if (e == hipSuccess)
+        e = hipMemcpy(o->view_dev, &o->view, sizeof(ObjectDev), hipMemcpyHostToDevice);
     if (e == hipSuccess)
         e = hipMemcpy(o->bvh, o->bvh_host.data(), sizeof(BvhNode) * o->bvh_host.size(), hipMemcpyHostToDevice);
     if (e == hipSuccess)
@@ -774,12 +741,7 @@ static int trace(const dmap_object *obj, const dmap_rays *rays, int64_t n, const
     HIP_TRY(hipSetDevice(obj->ctx->device));
     TraceParams p;
     std::memset(&p, 0, sizeof p);
-    p.pyr = view_of(obj->tile);
-    p.tris = obj->tris;
-    p.boxes = obj->boxes;
-    p.bvh = obj->bvh;
-    p.n_tri = obj->n_tri;
-    p.const_normal = obj->const_normal;
+    p.obj = obj->view;
     p.prim_mask = prim_mask;
     p.bound = o.bound;
     p.slab_max_level = o.slab_max_level;

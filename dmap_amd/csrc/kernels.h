@@ -53,7 +53,8 @@ struct BvhNode
     int parent, left, right, axis;
 };
 
-struct TraceParams
+// What the traversal reads of one object: its tile and its triangles.
+struct ObjectDev
 {
     PyramidView pyr;
     const TriRecord *tris;
@@ -61,6 +62,30 @@ struct TraceParams
     const BvhNode *bvh;
     int n_tri;
     int const_normal; // every triangle has a constant interpolated normal
+};
+
+// One request inside a multi-request launch (serving): rays [start,
+// start + count) of the launch's index space are rays 0..count-1 of
+// `rays`, traced against `obj`, their hits written to `hits`.
+struct Segment
+{
+    int64_t start, count;
+    const ObjectDev *obj;
+    RaysDev rays;
+    HitsDev hits;
+};
+constexpr int MAX_SEGMENTS = 256;
+struct StepDesc
+{
+    int64_t total;
+    int n_seg, pad_;
+    Segment seg[MAX_SEGMENTS];
+};
+
+struct TraceParams
+{
+    ObjectDev obj;        // single-object launches
+    const StepDesc *step; // multi-request launches (NULL for single-object ones)
     int prim_mask; // -1: every triangle; k: triangle k only
     int bound;     // 0 box, 1 slab, 2 both
     int slab_max_level;
@@ -85,5 +110,11 @@ hipError_t reorder_rays(const RaysDev &rays, int64_t n, const float lo[3], const
 // A closest-hit launch is followed by the finalize kernel, which turns the
 // raw record (t, prim, texel, texel-local root in s/tt) into the hit.
 hipError_t launch_trace(const TraceParams &p, int precision, hipStream_t stream);
+// The multi-request closest hit of serving: reset, trace and finalize over
+// the step descriptor at p.step (fp32, Newton leaf, no counters), with a
+// grid that does not depend on the step, so the three launches can be
+// captured once in a graph and replayed with new descriptors. Every object
+// of the step has constant normals (const_normal) or none has.
+hipError_t launch_trace_multi(const TraceParams &p, bool const_normal, hipStream_t stream);
 
 } // namespace dmap_amd

@@ -268,10 +268,9 @@ __device__ inline float4 fetch_tp(const PyramidView &P, int level, int i, int j)
 // Taylor quad tp. Returns whether the ray can meet the surface over the
 // cell within [t_min, t_max].
 template <class R, bool CN>
-__device__ bool cell_test(const TraceParams &p, const TriRecord &T, const Walk<R> &wk, int level, int ci, int cj,
-                          float2 mm, float4 tp, R t_min, R t_max)
+__device__ bool cell_test(const TraceParams &p, const PyramidView &P, const TriRecord &T, const Walk<R> &wk, int level,
+                          int ci, int cj, float2 mm, float4 tp, R t_min, R t_max)
 {
-    const PyramidView &P = p.pyr;
     R w = ldexp((R)1, level) / (R)P.n_leaf;
     R clo_x = (R)ci * w, clo_y = (R)cj * w;
     // The cell clipped to the triangle's texture bounding box (node_bounds.cpp:94-99).
@@ -445,10 +444,9 @@ __device__ inline float node_value(const PyramidView &P, int i, int j)
 }
 
 template <class R>
-__device__ void leaf_setup(const TraceParams &p, const TriRecord &T, const double o[3], const double d[3], int i,
+__device__ void leaf_setup(const PyramidView &P, const TriRecord &T, const double o[3], const double d[3], int i,
                            int j, Leaf<R> &L)
 {
-    const PyramidView &P = p.pyr;
     double w = 1.0 / P.n_leaf;
     double slo = i * w, tlo = j * w;
     // Corner heights h = scale * value + offset (HeightGrid::h at the corners).
@@ -1022,10 +1020,9 @@ __device__ inline float4 group_tp(const Group &g, int q)
 // them: bit k is cell (bi + ((k >> 1) ^ si), bj + ((k & 1) ^ sj)), the
 // child order of cell_next (displaced_intersector.cpp:99-121).
 template <class R, bool CN>
-__device__ uint32_t expand(const TraceParams &p, const TriRecord &T, const Walk<R> &wk, int level, int bi, int bj,
-                           bool aligned, uint32_t valid, R t_min, R t_max, int &node_tests)
+__device__ uint32_t expand(const TraceParams &p, const PyramidView &P, const TriRecord &T, const Walk<R> &wk, int level,
+                           int bi, int bj, bool aligned, uint32_t valid, R t_min, R t_max, int &node_tests)
 {
-    const PyramidView &P = p.pyr;
     bool slab = p.bound != 0 && level <= p.slab_max_level;
     Group g;
     if (aligned) {
@@ -1054,7 +1051,7 @@ __device__ uint32_t expand(const TraceParams &p, const TriRecord &T, const Walk<
             continue;
         ++node_tests;
         float4 tp = slab ? g.tp[q] : make_float4(0.f, 0.f, 0.f, 0.f);
-        if (cell_test<R, CN>(p, T, wk, level, ci, cj, group_mm(g, q), tp, t_min, t_max))
+        if (cell_test<R, CN>(p, P, T, wk, level, ci, cj, group_mm(g, q), tp, t_min, t_max))
             mask |= 1u << (((io ^ wk.si) << 1) | (jo ^ wk.sj));
     }
     return mask;
@@ -1087,7 +1084,7 @@ constexpr int REFILL = 16;
 #define DMAP_TRACE_WAVES 3
 #endif
 
-template <class R, bool STATS, bool CN, bool TWO>
+template <class R, bool STATS, bool CN, bool TWO, bool MULTI>
 __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(DMAP_TRACE_WAVES, 8)))
 k_trace(TraceParams p)
 {
@@ -1101,6 +1098,13 @@ k_trace(TraceParams p)
     // drained and its lanes are done.
     bool drained = false, have = false;
     int64_t rid = 0;
+    // Multi-request launches: the lane's request and its object.
+    const Segment *SG = nullptr;
+    const ObjectDev *O = nullptr;
+    const int64_t n_total = MULTI ? p.step->total : p.n_rays;
+#define DMAP_OBJ (MULTI ? *O : p.obj)
+#define DMAP_RAYS (MULTI ? SG->rays : p.rays)
+#define DMAP_HITS (MULTI ? SG->hits : p.hits)
     R t_min = 0, t_best = 0;
     // The ray as given (fp32, so exact in R and in double).
     R o_f[3] = {0, 0, 0}, d_f[3] = {0, 0, 1}, inv_f[3] = {0, 0, 0};
@@ -1113,7 +1117,7 @@ k_trace(TraceParams p)
     R hit_a = 0, hit_b = 0;
     LaneStats st{0, 0, 0, 0, 0, 0};
     Walk<R> wk;
-    const TriRecord *T = p.tris;
+    const TriRecord *T = p.obj.tris;
     int root_level = 0, rb_i = 0, rb_j = 0;
     int lvl = 0, pi = 0, pj = 0; // the group being walked: the cells at lvl below (pi, pj), or the root block
     uint64_t trail = 0;
@@ -1129,7 +1133,7 @@ k_trace(TraceParams p)
                 p.occluded[rid] = hit_tri >= 0 ? 1 : 0;
             } else {
                 // The raw record; k_finalize turns it into the hit.
-                const HitsDev &H = p.hits;
+                const HitsDev &H = DMAP_HITS;
                 H.t[rid] = hit_tri >= 0 ? (float)t_best : INFINITY;
                 H.prim[rid] = hit_tri;
                 H.texel_i[rid] = hit_i;
@@ -1163,21 +1167,37 @@ k_trace(TraceParams p)
                 if (lane == leader)
                     base = atomicAdd(p.queue, (unsigned)cnt);
                 base = __shfl(base, leader);
-                if ((int64_t)base + cnt >= p.n_rays)
+                if ((int64_t)base + cnt >= n_total)
                     drained = true;
                 if (!have) {
                     int64_t slot = (int64_t)base + __popcll(idle & ((1ull << lane) - 1));
-                    if (slot < p.n_rays) {
+                    if (slot < n_total) {
                         have = true;
-                        rid = p.perm ? (int64_t)p.perm[slot] : slot;
-                        o_f[0] = p.rays.ox[rid];
-                        o_f[1] = p.rays.oy[rid];
-                        o_f[2] = p.rays.oz[rid];
-                        d_f[0] = p.rays.dx[rid];
-                        d_f[1] = p.rays.dy[rid];
-                        d_f[2] = p.rays.dz[rid];
-                        t_min = (R)p.rays.tmin[rid];
-                        t_best = (R)p.rays.tmax[rid];
+                        if (MULTI) {
+                            // The request holding this slot: a binary search of the starts.
+                            int lo = 0, hi = p.step->n_seg - 1;
+                            while (lo < hi) {
+                                int mid = (lo + hi + 1) >> 1;
+                                if (p.step->seg[mid].start <= slot)
+                                    lo = mid;
+                                else
+                                    hi = mid - 1;
+                            }
+                            SG = &p.step->seg[lo];
+                            O = SG->obj;
+                            rid = slot - SG->start;
+                        } else {
+                            rid = p.perm ? (int64_t)p.perm[slot] : slot;
+                        }
+                        const RaysDev &RY = DMAP_RAYS;
+                        o_f[0] = RY.ox[rid];
+                        o_f[1] = RY.oy[rid];
+                        o_f[2] = RY.oz[rid];
+                        d_f[0] = RY.dx[rid];
+                        d_f[1] = RY.dy[rid];
+                        d_f[2] = RY.dz[rid];
+                        t_min = (R)RY.tmin[rid];
+                        t_best = (R)RY.tmax[rid];
                         for (int q = 0; q < 3; ++q)
                             inv_f[q] = (R)1 / d_f[q];
                         // A ray with a non-finite or zero component, or an empty
@@ -1241,12 +1261,12 @@ k_trace(TraceParams p)
                 k = -1;
                 if (!single_done) {
                     single_done = true;
-                    const float *bx = p.boxes + 6 * p.prim_mask;
+                    const float *bx = DMAP_OBJ.boxes + 6 * p.prim_mask;
                     if (ray_box<R>(o_f, d_f, inv_f, bx, bx + 3, t_min, t_best))
                         k = p.prim_mask;
                 }
             } else {
-                k = bvh_next<R>(p.bvh, bvh_cur, bvh_from, o_f, d_f, inv_f, t_min, t_best);
+                k = bvh_next<R>(DMAP_OBJ.bvh, bvh_cur, bvh_from, o_f, d_f, inv_f, t_min, t_best);
             }
             if (k < 0) {
                 state = ST_DONE;
@@ -1254,7 +1274,7 @@ k_trace(TraceParams p)
             }
             if (STATS)
                 ++st.tri;
-            T = p.tris + k;
+            T = DMAP_OBJ.tris + k;
             // The ray in tangent space (DisplacedIntersector::intersect :178),
             // formed in double.
             double rel[3] = {(double)o_f[0] - T->q0[0], (double)o_f[1] - T->q0[1], (double)o_f[2] - T->q0[2]};
@@ -1288,7 +1308,8 @@ k_trace(TraceParams p)
         }
         if (state == ST_EXPAND) {
             int tests = 0;
-            uint32_t mask = expand<R, CN>(p, *T, wk, ex_level, ex_i, ex_j, ex_aligned, ex_valid, t_min, t_best, tests);
+            uint32_t mask =
+                expand<R, CN>(p, DMAP_OBJ.pyr, *T, wk, ex_level, ex_i, ex_j, ex_aligned, ex_valid, t_min, t_best, tests);
             if (STATS)
                 st.node += tests;
             if (!ex_aligned) {
@@ -1314,7 +1335,7 @@ k_trace(TraceParams p)
                 const double o[3] = {(double)o_f[0], (double)o_f[1], (double)o_f[2]};
                 const double d[3] = {(double)d_f[0], (double)d_f[1], (double)d_f[2]};
                 Leaf<R> L;
-                leaf_setup<R>(p, *T, o, d, leaf_i, leaf_j, L);
+                leaf_setup<R>(DMAP_OBJ.pyr, *T, o, d, leaf_i, leaf_j, L);
                 R la = 0, lb = 0;
                 bool found;
                 if (!TWO)
@@ -1339,12 +1360,9 @@ k_trace(TraceParams p)
 // crossing recomputed (two-triangle leaf), then st, the chart's
 // barycentrics and ng = unit (Su x Sv) at the root
 // (displaced_intersector.cpp:357-366, 413-421).
-__global__ void __launch_bounds__(256) k_finalize(TraceParams p)
+__device__ void finalize_one(const TraceParams &p, const PyramidView &P, const TriRecord *tris, const RaysDev &rays,
+                             const HitsDev &H, int64_t rid)
 {
-    int64_t rid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (rid >= p.n_rays)
-        return;
-    const HitsDev &H = p.hits;
     int tri = H.prim[rid];
     if (tri < 0) {
         H.s[rid] = 0.f;
@@ -1363,13 +1381,13 @@ __global__ void __launch_bounds__(256) k_finalize(TraceParams p)
             H.nz[rid] = 0.f;
         return;
     }
-    double o[3] = {p.rays.ox[rid], p.rays.oy[rid], p.rays.oz[rid]};
-    double d[3] = {p.rays.dx[rid], p.rays.dy[rid], p.rays.dz[rid]};
-    double t_min = p.rays.tmin[rid];
-    const TriRecord &TH = p.tris[tri];
+    double o[3] = {rays.ox[rid], rays.oy[rid], rays.oz[rid]};
+    double d[3] = {rays.dx[rid], rays.dy[rid], rays.dz[rid]};
+    double t_min = rays.tmin[rid];
+    const TriRecord &TH = tris[tri];
     int hi = H.texel_i[rid], hj = H.texel_j[rid];
     Leaf<double> L;
-    leaf_setup<double>(p, TH, o, d, hi, hj, L);
+    leaf_setup<double>(P, TH, o, d, hi, hj, L);
     double a = H.s[rid], b = H.tt[rid], t_hit = H.t[rid];
     double Sa[3], Sb[3];
     if (p.leaf == 1) {
@@ -1440,6 +1458,33 @@ __global__ void __launch_bounds__(256) k_finalize(TraceParams p)
     if (H.nz)
         H.nz[rid] = (float)n[2];
 }
+
+template <bool MULTI> __global__ void __launch_bounds__(256) k_finalize(TraceParams p)
+{
+    int64_t rid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const Segment *SG = nullptr;
+    if (MULTI) {
+        // Grid-stride over the step's requests (the grid is fixed for graph replay).
+        const StepDesc &S = *p.step;
+        for (int64_t slot = rid; slot < S.total; slot += (int64_t)gridDim.x * blockDim.x) {
+            int lo = 0, hi = S.n_seg - 1;
+            while (lo < hi) {
+                int mid = (lo + hi + 1) >> 1;
+                if (S.seg[mid].start <= slot)
+                    lo = mid;
+                else
+                    hi = mid - 1;
+            }
+            SG = &S.seg[lo];
+            finalize_one(p, SG->obj->pyr, SG->obj->tris, SG->rays, SG->hits, slot - SG->start);
+        }
+        return;
+    }
+    if (rid >= p.n_rays)
+        return;
+    finalize_one(p, p.obj.pyr, p.obj.tris, p.rays, p.hits, rid);
+}
+
 
 // ------------------------------------------------------- height range
 
@@ -1515,10 +1560,22 @@ __global__ void k_queue_reset(unsigned *queue) { *queue = 0u; }
 template <class R, bool STATS, bool TWO>
 static void launch_trace_l(const TraceParams &p, unsigned blocks, hipStream_t stream)
 {
-    if (p.const_normal)
-        hipLaunchKernelGGL((k_trace<R, STATS, true, TWO>), dim3(blocks), dim3(BLOCK), 0, stream, p);
+    if (p.obj.const_normal)
+        hipLaunchKernelGGL((k_trace<R, STATS, true, TWO, false>), dim3(blocks), dim3(BLOCK), 0, stream, p);
     else
-        hipLaunchKernelGGL((k_trace<R, STATS, false, TWO>), dim3(blocks), dim3(BLOCK), 0, stream, p);
+        hipLaunchKernelGGL((k_trace<R, STATS, false, TWO, false>), dim3(blocks), dim3(BLOCK), 0, stream, p);
+}
+
+static int cu_count()
+{
+    static int cus = 0;
+    if (cus == 0) {
+        int dev = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
+            cus = 256;
+    }
+    return cus;
 }
 
 template <class R, bool STATS>
@@ -1535,14 +1592,7 @@ hipError_t launch_trace(const TraceParams &p, int precision, hipStream_t stream)
     if (p.n_rays <= 0)
         return hipSuccess;
     // Persistent waves: enough workgroups to fill every CU a few times over.
-    static int cus = 0;
-    if (cus == 0) {
-        int dev = 0;
-        hipGetDevice(&dev);
-        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-        if (cus <= 0)
-            cus = 256;
-    }
+    int cus = cu_count();
     int64_t want = (p.n_rays + BLOCK - 1) / BLOCK;
     unsigned blocks = (unsigned)(want < (int64_t)cus * 4 ? want : (int64_t)cus * 4);
     hipLaunchKernelGGL(k_queue_reset, dim3(1), dim3(1), 0, stream, p.queue);
@@ -1563,7 +1613,22 @@ hipError_t launch_trace(const TraceParams &p, int precision, hipStream_t stream)
     if (e != hipSuccess || p.any_hit)
         return e;
     unsigned fblocks = (unsigned)((p.n_rays + 255) / 256);
-    hipLaunchKernelGGL(k_finalize, dim3(fblocks), dim3(256), 0, stream, p);
+    hipLaunchKernelGGL(k_finalize<false>, dim3(fblocks), dim3(256), 0, stream, p);
+    return hipGetLastError();
+}
+
+hipError_t launch_trace_multi(const TraceParams &p, bool const_normal, hipStream_t stream)
+{
+    unsigned blocks = (unsigned)cu_count() * 4;
+    hipLaunchKernelGGL(k_queue_reset, dim3(1), dim3(1), 0, stream, p.queue);
+    // A serving step holds objects of one kind only (constant normals or
+    // interpolated ones), so it runs the same specialisation a single-object
+    // launch of those objects would.
+    if (const_normal)
+        hipLaunchKernelGGL((k_trace<float, false, true, false, true>), dim3(blocks), dim3(BLOCK), 0, stream, p);
+    else
+        hipLaunchKernelGGL((k_trace<float, false, false, false, true>), dim3(blocks), dim3(BLOCK), 0, stream, p);
+    hipLaunchKernelGGL(k_finalize<true>, dim3(blocks * 2), dim3(256), 0, stream, p);
     return hipGetLastError();
 }
 

@@ -133,6 +133,30 @@ int dmap_intersect_closest(const dmap_object *obj, const dmap_rays *rays, int64_
 int dmap_intersect_any(const dmap_object *obj, const dmap_rays *rays, int64_t n, uint8_t *occluded, int prim_mask,
                        const dmap_options *options, void *stream);
 
+/* Serving (continuous batching; no counterpart in the reference, which
+ * traces one ray per call). Requests against resident objects are queued,
+ * admitted in arrival order into the next free slot while the step's ray
+ * budget and the hit pool allow, and traced by one persistent multi-request
+ * launch per step (replayed from a captured HIP graph). Up to n_slots steps
+ * overlap on the device. Ray buffers stay owned by the caller and must
+ * stay valid until the request completes; hits live in the server's pool
+ * (44 B per ray) until released. */
+typedef struct dmap_server dmap_server;
+int dmap_server_create(dmap_ctx *ctx, int n_slots, int64_t step_rays, int64_t hit_pool_bytes, dmap_server **out);
+int dmap_server_submit(dmap_server *server, const dmap_object *obj, const dmap_rays *rays, int64_t n, uint64_t tag);
+/* Completed requests since the last poll: their tags and latencies
+ * (submission to completion, seconds). */
+int dmap_server_poll(dmap_server *server, int max, uint64_t *tags, double *latency_s, int *n_done);
+/* The device hit arrays of a completed request (valid until released). */
+int dmap_server_hits(dmap_server *server, uint64_t tag, dmap_hits *hits);
+/* Copy a completed request's hits into caller-owned device arrays (NULL
+ * fields are skipped), asynchronously on `stream`. */
+int dmap_server_copy_hits(dmap_server *server, uint64_t tag, const dmap_hits *dst, void *stream);
+int dmap_server_release(dmap_server *server, uint64_t tag);
+/* idle = 1 when nothing is queued or in flight. */
+int dmap_server_idle(dmap_server *server, int *idle);
+void dmap_server_destroy(dmap_server *server);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,135 @@
+"""The serving loop: every request's hits are the same alone and in the mix,
+and the same as the single-request entry point's; chunked requests, steps
+replayed from the captured graphs many times with eager work between them,
+memory-based admission, and objects of both normal kinds in one server."""
+import numpy as np
+import pytest
+
+from dmap_amd import serving
+from dmap_amd import workload as W
+from tests import scenes as S
+
+pytestmark = pytest.mark.gpu
+
+FIELDS = ("t", "st", "bary", "ng", "prim", "texel")
+
+
+def _host(h):
+    return {k: h[k].cpu().numpy() for k in FIELDS}
+
+
+def _same(a, b):
+    return all(np.array_equal(a[k], b[k]) for k in FIELDS)
+
+
+def _run(server, jobs, release=True):
+    """Submit every (obj, rays) at once, wait for all; returns host hits by index."""
+    import torch
+    for q, (obj, rays) in enumerate(jobs):
+        server.submit(obj, rays, q)
+    out, lat = {}, {}
+    while len(out) < len(jobs):
+        for tag, l in server.poll():
+            out[tag] = _host(server.hits(tag, jobs[tag][1].shape[1]))
+            lat[tag] = l
+            if release:
+                server.release(tag)
+    torch.cuda.synchronize()
+    assert server.idle()
+    assert all(l >= 0 for l in lat.values())
+    return out
+
+
+@pytest.fixture(scope="module")
+def small_scenes(torch_gpu):
+    print("[serving] building 3 scenes (128^2, 256^2, 512^2, pools of 16384 rays)", flush=True)
+    return serving.make_scenes((128, 256, 512), pool=1 << 14)
+
+
+def test_mix_equals_alone_and_plain(small_scenes):
+    import dmap_amd
+    scn = small_scenes
+    reqs = serving.make_requests(scn, 40, seed=11, min_rays=100, max_rays=12000)
+    jobs = [(scn[k].obj, r) for k, r in reqs]
+    # A small step budget splits the larger requests over several steps.
+    srv = dmap_amd.Server(slots=3, step_rays=8192, hit_pool_bytes=64 << 20)
+    mix = _run(srv, jobs)
+    for q, (obj, rays) in enumerate(jobs):
+        alone = _run(srv, [(obj, rays)])[0]
+        assert _same(mix[q], alone), f"request {q}: mix and alone differ"
+        plain = _host(obj.intersect(rays))
+        assert _same(mix[q], plain), f"request {q}: server and single-request entry point differ"
+        if q % 8 == 0:
+            # eager work on the default stream between graph replays
+            obj.intersect(rays, options=dmap_amd.TraversalOptions(reorder=False))
+    hit = np.concatenate([np.isfinite(m["t"]) for m in mix.values()])
+    assert 0.2 < hit.mean() < 1.0
+
+
+def test_memory_admission(small_scenes):
+    """A hit pool that holds ~3 requests at once: the rest wait for releases."""
+    import dmap_amd
+    scn = small_scenes
+    reqs = serving.make_requests(scn, 12, seed=5, min_rays=4000, max_rays=9000)
+    jobs = [(scn[k].obj, r) for k, r in reqs]
+    srv = dmap_amd.Server(slots=2, step_rays=1 << 20, hit_pool_bytes=44 * 30000)
+    mix = _run(srv, jobs)
+    for q, (obj, rays) in enumerate(jobs):
+        assert _same(mix[q], _host(obj.intersect(rays))), f"request {q}"
+    import torch
+    big = torch.cat([scn[0].primary, scn[0].bounce], dim=1).contiguous()  # 32768 rays: more than the pool holds
+    with pytest.raises(dmap_amd.DmapError):
+        srv.submit(scn[0].obj, big, 999)
+
+
+def test_unreleased_hits_block_admission(small_scenes):
+    """Hits stay valid until released; admission waits for the pool."""
+    import torch
+
+    import dmap_amd
+    scn = small_scenes
+    rays = [scn[0].primary[:, 64 * k:64 * k + 5000].contiguous() for k in range(3)]
+    srv = dmap_amd.Server(slots=2, step_rays=1 << 20, hit_pool_bytes=44 * 11000)
+    srv.submit(scn[0].obj, rays[0], 0)
+    srv.submit(scn[0].obj, rays[1], 1)
+    srv.submit(scn[0].obj, rays[2], 2)
+    done = set()
+    while len(done) < 2:
+        done |= {t for t, _ in srv.poll()}
+    torch.cuda.synchronize()
+    assert done == {0, 1}
+    assert not srv.idle()  # the third waits for room
+    h0 = _host(srv.hits(0, 5000))
+    srv.release(0)
+    while 2 not in done:
+        done |= {t for t, _ in srv.poll()}
+    assert _same(h0, _host(scn[0].obj.intersect(rays[0])))
+    assert _same(_host(srv.hits(2, 5000)), _host(scn[0].obj.intersect(rays[2])))
+    with pytest.raises(dmap_amd.DmapError):
+        srv.submit(scn[0].obj, rays[0], 1)  # tag 1 is still held
+    srv.release(1)
+    srv.release(2)
+    with pytest.raises(dmap_amd.DmapError):
+        srv.release(2)
+
+
+def test_both_normal_kinds(small_scenes, torch_gpu):
+    """Interpolated-normal and constant-normal objects share one server."""
+    import dmap_amd
+    torch = torch_gpu
+    tex = np.random.default_rng(3).random((64, 64)).astype(np.float32)
+    tile = dmap_amd.DisplacementTile(tex, 0.05)
+    curved = dmap_amd.ObjectIntersector(tile, S.curved_quad())
+    o, d = W.pinhole_rays((0.5, -0.8, 0.9), (0.5, 0.5, 0.0), 60.0, 64, 64)
+    rc = torch.from_numpy(W.pack_rays(o, d)).cuda()
+    scn = small_scenes
+    jobs = []
+    for q in range(10):
+        if q % 2:
+            jobs.append((curved, rc[:, 256 * q:256 * q + 1500].contiguous()))
+        else:
+            jobs.append((scn[q % 3].obj, scn[q % 3].bounce[:, 1000 * q:1000 * q + 3000].contiguous()))
+    srv = dmap_amd.Server(slots=2, step_rays=4096, hit_pool_bytes=16 << 20)
+    mix = _run(srv, jobs)
+    for q, (obj, rays) in enumerate(jobs):
+        assert _same(mix[q], _host(obj.intersect(rays))), f"request {q}"
