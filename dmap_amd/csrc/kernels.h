@@ -64,17 +64,18 @@ struct ObjectDev
     int const_normal; // every triangle has a constant interpolated normal
 };
 
-// One request inside a multi-request launch (serving): rays [start,
-// start + count) of the launch's index space are rays 0..count-1 of
-// `rays`, traced against `obj`, their hits written to `hits`.
+// One request's share of a serving step: `count` rays at step slots
+// [start, start + count), which are the request's rays first .. first +
+// count (rays and hits point at the request's arrays).
 struct Segment
 {
-    int64_t start, count;
+    int64_t start, count, first;
     const ObjectDev *obj;
     RaysDev rays;
     HitsDev hits;
 };
 constexpr int MAX_SEGMENTS = 256;
+constexpr int64_t MAX_STEP_RAYS = (int64_t)1 << 30;
 struct StepDesc
 {
     int64_t total;

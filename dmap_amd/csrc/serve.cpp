@@ -22,12 +22,13 @@
 // captured once per slot and kind as a HIP graph and replayed with the
 // step's descriptor, so requests that arrive while others run are admitted
 // into the next free slot and overlap on the device with the steps already
-// running. A host callback stamps each step's completion time; a request's
-// latency is the completion of its last step minus its submission.
+// running. Each step records an event; the server's calls (submit, poll,
+// release, idle) query the events, and a request's latency is the time its
+// last step was seen complete minus its submission, so a caller that polls
+// continuously measures it to within one poll.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
-#include <atomic>
 #include <chrono>
 #include <cstring>
 #include <map>
@@ -95,6 +96,8 @@ struct Arena
 
 // Requests waiting longer than this go first, in arrival order.
 constexpr double AGING_S = 0.02;
+// Hit pool bytes per ray: 8 fp32 + 3 int32 hit fields.
+constexpr size_t POOL_BYTES_PER_RAY = 44;
 
 struct Request
 {
@@ -119,17 +122,10 @@ struct Slot
     StepDesc *desc_dev = nullptr;
     unsigned *queue = nullptr;
     hipGraphExec_t graph[2] = {nullptr, nullptr}; // by kind: interpolated, constant normals
+    hipEvent_t done = nullptr;                     // recorded after the step's launches
     std::vector<Request *> reqs;
-    std::atomic<int> busy{0};
-    std::atomic<double> t_done{0.0};
+    bool busy = false;
 };
-
-void CL_stamp(void *arg)
-{
-    Slot *s = (Slot *)arg;
-    s->t_done.store(now_s());
-    s->busy.store(2); // finished, not yet collected
-}
 
 } // namespace
 
@@ -176,25 +172,23 @@ HitsDev hits_dev(const dmap_hits &h)
     return HitsDev{h.t, h.s, h.tt, h.bu, h.bv, h.nx, h.ny, h.nz, h.prim, h.texel_i, h.texel_j};
 }
 
-// A request's arrays offset by `k` rays.
-RaysDev rays_at(const dmap_rays &r, int64_t k)
+RaysDev rays_dev(const dmap_rays &r)
 {
-    return RaysDev{r.ox + k, r.oy + k, r.oz + k, r.dx + k, r.dy + k, r.dz + k, r.tmin + k, r.tmax + k};
-}
-
-HitsDev hits_at(const dmap_hits &h, int64_t k)
-{
-    return HitsDev{h.t + k,  h.s + k,  h.tt + k, h.bu + k,   h.bv + k,      h.nx + k,
-                   h.ny + k, h.nz + k, h.prim + k, h.texel_i + k, h.texel_j + k};
+    return RaysDev{r.ox, r.oy, r.oz, r.dx, r.dy, r.dz, r.tmin, r.tmax};
 }
 
 // Collect finished slots; then fill every free slot with a new step.
 int pump(dmap_server *S)
 {
     for (Slot *sl : S->slots) {
-        if (sl->busy.load() != 2)
+        if (!sl->busy)
             continue;
-        double t = sl->t_done.load();
+        hipError_t q = hipEventQuery(sl->done);
+        if (q == hipErrorNotReady)
+            continue;
+        if (q != hipSuccess)
+            return fail(DMAP_ERR_HIP, "serving step: %s", hipGetErrorString(q));
+        const double t = now_s();
         for (Request *r : sl->reqs) {
             if (--r->inflight == 0 && r->next == r->n) {
                 r->t_done = t;
@@ -204,11 +198,11 @@ int pump(dmap_server *S)
             }
         }
         sl->reqs.clear();
-        sl->busy.store(0);
+        sl->busy = false;
     }
     const double now = now_s();
     for (Slot *sl : S->slots) {
-        if (sl->busy.load() != 0 || S->pending.empty())
+        if (sl->busy || S->pending.empty())
             continue;
         std::vector<Request *> order(S->pending);
         std::sort(order.begin(), order.end(), [now](const Request *a, const Request *b) {
@@ -232,7 +226,7 @@ int pump(dmap_server *S)
             if ((r->obj->const_normal ? 1 : 0) != kind)
                 continue;
             if (!r->admitted) {
-                size_t bytes = (size_t)r->n * 44;
+                size_t bytes = (size_t)r->n * POOL_BYTES_PER_RAY;
                 size_t off;
                 if (!S->arena.alloc(bytes, off)) {
                     if (now - r->t_submit > AGING_S)
@@ -248,9 +242,10 @@ int pump(dmap_server *S)
             Segment &g = D.seg[n_seg++];
             g.start = total;
             g.count = take;
+            g.first = r->next;
             g.obj = r->obj->view_dev;
-            g.rays = rays_at(r->rays, r->next);
-            g.hits = hits_at(r->hits, r->next);
+            g.rays = rays_dev(r->rays);
+            g.hits = hits_dev(r->hits);
             r->next += take;
             ++r->inflight;
             total += take;
@@ -264,10 +259,10 @@ int pump(dmap_server *S)
         D.total = total;
         D.n_seg = n_seg;
         size_t desc_bytes = offsetof(StepDesc, seg) + sizeof(Segment) * n_seg;
-        sl->busy.store(1);
+        sl->busy = true;
         HIP_TRY(hipMemcpyAsync(sl->desc_dev, sl->desc_host, desc_bytes, hipMemcpyHostToDevice, sl->stream));
         HIP_TRY(hipGraphLaunch(sl->graph[kind], sl->stream));
-        HIP_TRY(hipLaunchHostFunc(sl->stream, CL_stamp, sl));
+        HIP_TRY(hipEventRecord(sl->done, sl->stream));
     }
     return DMAP_OK;
 }
@@ -278,7 +273,8 @@ extern "C" {
 
 int dmap_server_create(dmap_ctx *ctx, int n_slots, int64_t step_rays, int64_t hit_pool_bytes, dmap_server **out)
 {
-    if (!ctx || !out || n_slots < 1 || n_slots > 16 || step_rays < 1 || hit_pool_bytes < 4096)
+    if (!ctx || !out || n_slots < 1 || n_slots > 16 || step_rays < 1 || step_rays > MAX_STEP_RAYS ||
+        hit_pool_bytes < 4096)
         return fail(DMAP_ERR_INVALID, "dmap_server_create: bad argument");
     HIP_TRY(hipSetDevice(ctx->device));
     dmap_server *S = new dmap_server;
@@ -301,6 +297,8 @@ int dmap_server_create(dmap_ctx *ctx, int n_slots, int64_t step_rays, int64_t hi
             e = hipMalloc(&sl->desc_dev, sizeof(StepDesc));
         if (e == hipSuccess)
             e = hipMalloc(&sl->queue, 256);
+        if (e == hipSuccess)
+            e = hipEventCreateWithFlags(&sl->done, hipEventDisableTiming);
         if (e != hipSuccess)
             break;
         // Capture the step's three launches once; they read the descriptor
@@ -342,7 +340,7 @@ int dmap_server_submit(dmap_server *S, const dmap_object *obj, const dmap_rays *
 {
     if (!S || !obj || !rays || n <= 0)
         return fail(DMAP_ERR_INVALID, "dmap_server_submit: bad argument");
-    if ((size_t)n * 44 > S->arena.bytes)
+    if ((size_t)n * POOL_BYTES_PER_RAY > S->arena.bytes)
         return fail(DMAP_ERR_INVALID, "dmap_server_submit: %lld rays need more than the whole hit pool",
                     (long long)n);
     std::lock_guard<std::mutex> lk(S->mu);
@@ -445,7 +443,7 @@ int dmap_server_idle(dmap_server *S, int *idle)
     int rc = pump(S);
     bool any = !S->pending.empty();
     for (Slot *sl : S->slots)
-        any = any || sl->busy.load() != 0;
+        any = any || sl->busy;
     *idle = any ? 0 : 1;
     return rc;
 }
@@ -461,6 +459,8 @@ void dmap_server_destroy(dmap_server *S)
         for (hipGraphExec_t g : sl->graph)
             if (g)
                 (void)hipGraphExecDestroy(g);
+        if (sl->done)
+            (void)hipEventDestroy(sl->done);
         if (sl->stream)
             (void)hipStreamDestroy(sl->stream);
         (void)hipHostFree(sl->desc_host);

@@ -752,7 +752,7 @@ template <class R, bool CN>
 __device__ bool search_leaf(const TriRecord &T, const Leaf<R> &L, R t_min, R &t_max, R &hit_a, R &hit_b,
                             typename StackSlot<R>::T *s_t, uint16_t *s_code, LaneStats &st)
 {
-    const int lane = threadIdx.x;
+    const int lane = threadIdx.x, stride = blockDim.x;
     int sp = 0;
     bool found = false;
     uint32_t code = 0; // depth 0, the whole texel
@@ -802,8 +802,8 @@ __device__ bool search_leaf(const TriRecord &T, const Leaf<R> &L, R t_min, R &t_
 #undef DMAP_CSWAP
             for (int q = 3; q >= 0; --q)
                 if (key[q] != (R)INFINITY) {
-                    s_t[sp * BLOCK + lane] = key[q];
-                    s_code[sp * BLOCK + lane] = (uint16_t)cc[q];
+                    s_t[sp * stride + lane] = key[q];
+                    s_code[sp * stride + lane] = (uint16_t)cc[q];
                     ++sp;
                 }
         }
@@ -811,9 +811,9 @@ __device__ bool search_leaf(const TriRecord &T, const Leaf<R> &L, R t_min, R &t_
         bool have = false;
         while (sp > 0) {
             --sp;
-            R tl = s_t[sp * BLOCK + lane];
+            R tl = s_t[sp * stride + lane];
             if (tl < t_max) {
-                code = s_code[sp * BLOCK + lane];
+                code = s_code[sp * stride + lane];
                 have = true;
                 break;
             }
@@ -1088,8 +1088,12 @@ template <class R, bool STATS, bool CN, bool TWO, bool MULTI>
 __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(DMAP_TRACE_WAVES, 8)))
 k_trace(TraceParams p)
 {
-    __shared__ typename StackSlot<R>::T s_t[STACK * BLOCK];
-    __shared__ uint16_t s_code[STACK * BLOCK];
+
+    // The leaf search stacks: STACK entries per thread, stride blockDim.x
+    // (trace_lds_bytes per launch).
+    extern __shared__ __align__(16) unsigned char s_lds[];
+    typename StackSlot<R>::T *s_t = (typename StackSlot<R>::T *)s_lds;
+    uint16_t *s_code = (uint16_t *)(s_t + STACK * blockDim.x);
 
     const int lane = threadIdx.x & 63;
     // Persistent waves: a lane that has finished its ray writes the result
@@ -1185,7 +1189,7 @@ k_trace(TraceParams p)
                             }
                             SG = &p.step->seg[lo];
                             O = SG->obj;
-                            rid = slot - SG->start;
+                            rid = slot - SG->start + SG->first;
                         } else {
                             rid = p.perm ? (int64_t)p.perm[slot] : slot;
                         }
@@ -1360,11 +1364,13 @@ k_trace(TraceParams p)
 // crossing recomputed (two-triangle leaf), then st, the chart's
 // barycentrics and ng = unit (Su x Sv) at the root
 // (displaced_intersector.cpp:357-366, 413-421).
-__device__ void finalize_one(const TraceParams &p, const PyramidView &P, const TriRecord *tris, const RaysDev &rays,
-                             const HitsDev &H, int64_t rid)
+__device__ void finalize_hit(const TraceParams &p, const PyramidView &P, const TriRecord *tris, const RaysDev &rays,
+                             const HitsDev &H, int64_t rid, int tri, float t_raw, float a_raw, float b_raw, int hi,
+                             int hj)
 {
-    int tri = H.prim[rid];
+    H.prim[rid] = tri;
     if (tri < 0) {
+        H.t[rid] = INFINITY;
         H.s[rid] = 0.f;
         H.tt[rid] = 0.f;
         H.texel_i[rid] = 0;
@@ -1381,14 +1387,15 @@ __device__ void finalize_one(const TraceParams &p, const PyramidView &P, const T
             H.nz[rid] = 0.f;
         return;
     }
+    H.texel_i[rid] = hi;
+    H.texel_j[rid] = hj;
     double o[3] = {rays.ox[rid], rays.oy[rid], rays.oz[rid]};
     double d[3] = {rays.dx[rid], rays.dy[rid], rays.dz[rid]};
     double t_min = rays.tmin[rid];
     const TriRecord &TH = tris[tri];
-    int hi = H.texel_i[rid], hj = H.texel_j[rid];
     Leaf<double> L;
     leaf_setup<double>(P, TH, o, d, hi, hj, L);
-    double a = H.s[rid], b = H.tt[rid], t_hit = H.t[rid];
+    double a = a_raw, b = b_raw, t_hit = t_raw;
     double Sa[3], Sb[3];
     if (p.leaf == 1) {
         double ra, rb, tm = (double)INFINITY;
@@ -1459,6 +1466,14 @@ __device__ void finalize_one(const TraceParams &p, const PyramidView &P, const T
         H.nz[rid] = (float)n[2];
 }
 
+// The same from the raw record the trace left in the hit arrays.
+__device__ void finalize_one(const TraceParams &p, const PyramidView &P, const TriRecord *tris, const RaysDev &rays,
+                             const HitsDev &H, int64_t rid)
+{
+    finalize_hit(p, P, tris, rays, H, rid, H.prim[rid], H.t[rid], H.s[rid], H.tt[rid], H.texel_i[rid],
+                 H.texel_j[rid]);
+}
+
 template <bool MULTI> __global__ void __launch_bounds__(256) k_finalize(TraceParams p)
 {
     int64_t rid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1476,7 +1491,7 @@ template <bool MULTI> __global__ void __launch_bounds__(256) k_finalize(TracePar
                     hi = mid - 1;
             }
             SG = &S.seg[lo];
-            finalize_one(p, SG->obj->pyr, SG->obj->tris, SG->rays, SG->hits, slot - SG->start);
+            finalize_one(p, SG->obj->pyr, SG->obj->tris, SG->rays, SG->hits, slot - SG->start + SG->first);
         }
         return;
     }
@@ -1557,13 +1572,25 @@ hipError_t launch_height_range(const PyramidView &P, const TriRecord *tris, int 
 
 __global__ void k_queue_reset(unsigned *queue) { *queue = 0u; }
 
+template <class R> static size_t trace_lds_bytes(unsigned block)
+{
+    return (size_t)STACK * block * (sizeof(typename StackSlot<R>::T) + sizeof(uint16_t));
+}
+
+// Multi-request launches use one-wave workgroups: a workgroup's resources
+// free as soon as its wave is done, so the next step's waves fill the tail
+// of a running step wave by wave.
+constexpr unsigned MULTI_BLOCK = 64;
+
 template <class R, bool STATS, bool TWO>
 static void launch_trace_l(const TraceParams &p, unsigned blocks, hipStream_t stream)
 {
     if (p.obj.const_normal)
-        hipLaunchKernelGGL((k_trace<R, STATS, true, TWO, false>), dim3(blocks), dim3(BLOCK), 0, stream, p);
+        hipLaunchKernelGGL((k_trace<R, STATS, true, TWO, false>), dim3(blocks), dim3(BLOCK), trace_lds_bytes<R>(BLOCK),
+                           stream, p);
     else
-        hipLaunchKernelGGL((k_trace<R, STATS, false, TWO, false>), dim3(blocks), dim3(BLOCK), 0, stream, p);
+        hipLaunchKernelGGL((k_trace<R, STATS, false, TWO, false>), dim3(blocks), dim3(BLOCK), trace_lds_bytes<R>(BLOCK),
+                           stream, p);
 }
 
 static int cu_count()
@@ -1619,16 +1646,18 @@ hipError_t launch_trace(const TraceParams &p, int precision, hipStream_t stream)
 
 hipError_t launch_trace_multi(const TraceParams &p, bool const_normal, hipStream_t stream)
 {
-    unsigned blocks = (unsigned)cu_count() * 4;
+    // As many waves as the persistent single-object launch.
+    unsigned blocks = (unsigned)cu_count() * 4 * (BLOCK / MULTI_BLOCK);
+    const size_t lds = trace_lds_bytes<float>(MULTI_BLOCK);
     hipLaunchKernelGGL(k_queue_reset, dim3(1), dim3(1), 0, stream, p.queue);
     // A serving step holds objects of one kind only (constant normals or
     // interpolated ones), so it runs the same specialisation a single-object
     // launch of those objects would.
     if (const_normal)
-        hipLaunchKernelGGL((k_trace<float, false, true, false, true>), dim3(blocks), dim3(BLOCK), 0, stream, p);
+        hipLaunchKernelGGL((k_trace<float, false, true, false, true>), dim3(blocks), dim3(MULTI_BLOCK), lds, stream, p);
     else
-        hipLaunchKernelGGL((k_trace<float, false, false, false, true>), dim3(blocks), dim3(BLOCK), 0, stream, p);
-    hipLaunchKernelGGL(k_finalize<true>, dim3(blocks * 2), dim3(256), 0, stream, p);
+        hipLaunchKernelGGL((k_trace<float, false, false, false, true>), dim3(blocks), dim3(MULTI_BLOCK), lds, stream, p);
+    hipLaunchKernelGGL(k_finalize<true>, dim3(blocks), dim3(MULTI_BLOCK), 0, stream, p);
     return hipGetLastError();
 }
 
