@@ -9,6 +9,9 @@ reference's intersector interface (code/cpp/src/):
                         .intersect = closest hit (intersect_flat :578-599),
                         .occluded = any hit (displaced_geometry.cpp:79-106)
   TraversalOptions      TraversalOptions (traversal_options.h:30-38)
+  displaced_hits        differentiable closest hit (t, ng) with respect to the
+                        texels (no counterpart in the reference), backed by
+                        the adjoint kernel
 
 Device memory and streams are torch's (ROCm); every computation runs in the
 HIP kernels of the library. There is no CPU fallback: without the built
@@ -35,8 +38,10 @@ STAT_NAMES = ("triangle_tests", "node_tests", "leaf_tests", "sub_squares", "newt
 EXPORTS = ("dmap_last_error", "dmap_version", "dmap_ctx_create", "dmap_ctx_destroy", "dmap_tile_create",
            "dmap_tile_rebuild", "dmap_tile_info", "dmap_tile_read_level", "dmap_tile_read_nodes",
            "dmap_tile_destroy", "dmap_object_create", "dmap_object_triangle_info", "dmap_object_destroy",
-           "dmap_intersect_closest", "dmap_intersect_any", "dmap_server_create", "dmap_server_submit",
-           "dmap_server_poll", "dmap_server_hits", "dmap_server_copy_hits", "dmap_server_release", "dmap_server_idle", "dmap_server_destroy")
+           "dmap_intersect_closest", "dmap_intersect_any", "dmap_tile_set_texels", "dmap_object_refresh",
+           "dmap_intersect_adjoint", "dmap_server_create", "dmap_server_submit", "dmap_server_poll",
+           "dmap_server_hits", "dmap_server_copy_hits", "dmap_server_release", "dmap_server_idle",
+           "dmap_server_destroy")
 
 
 class DmapError(RuntimeError):
@@ -92,6 +97,10 @@ def load_library():
         L.dmap_intersect_closest.argtypes = [vp, c.POINTER(_Rays), c.c_int64, c.POINTER(_Hits), c.c_int,
                                              c.POINTER(_Options), c.POINTER(_Stats), vp]
         L.dmap_intersect_any.argtypes = [vp, c.POINTER(_Rays), c.c_int64, vp, c.c_int, c.POINTER(_Options), vp]
+        L.dmap_tile_set_texels.argtypes = [vp, vp, c.c_int, vp]
+        L.dmap_object_refresh.argtypes = [vp]
+        L.dmap_intersect_adjoint.argtypes = [vp, c.POINTER(_Rays), c.c_int64, vp, vp, vp, vp, vp,
+                                             c.POINTER(_Options), vp]
         _lib = L
     return _lib
 
@@ -185,6 +194,22 @@ class DisplacementTile:
     def rebuild(self, stream=None):
         _check(_lib.dmap_tile_rebuild(self._h, _stream_ptr(stream)), "dmap_tile_rebuild")
 
+    def set_texels(self, texels, stream=None):
+        """New texel values (same n x n): closes the grid and rebuilds the
+        pyramid. Objects over the tile need refresh() afterwards."""
+        torch = _torch()
+        if isinstance(texels, torch.Tensor) and texels.is_cuda:
+            t = texels.detach().to(torch.float32).contiguous()
+            on_device, ptr = 1, t.data_ptr()
+        else:
+            t = np.ascontiguousarray(texels, dtype=np.float32)
+            on_device, ptr = 0, t.ctypes.data
+        if tuple(t.shape) != (self.n_leaf, self.n_leaf):
+            raise DmapError(f"texels must be {self.n_leaf} x {self.n_leaf}")
+        _check(_lib.dmap_tile_set_texels(self._h, ctypes.c_void_p(ptr), on_device, _stream_ptr(stream)),
+               "dmap_tile_set_texels")
+        self._keep = t
+
     def level(self, k):
         """Channels of level k as a dict of (m, m) float32 arrays [j, i]."""
         m = self.n_leaf >> k
@@ -267,6 +292,41 @@ class ObjectIntersector:
         if stats:
             res["stats"] = {k: stats_t[q] for q, k in enumerate(STAT_NAMES)}
         return res
+
+    def refresh(self):
+        """Triangle bounds and BVH from the tile's current pyramid."""
+        _check(_lib.dmap_object_refresh(self._h), "dmap_object_refresh")
+
+    def adjoint(self, rays, dl_dt=None, dl_dn=None, grad=None, options=None, stream=None):
+        """dL/d(texel) (n_leaf x n_leaf float64 device tensor, added into
+        `grad` when given) for dL/dt (n,) and dL/dng (3, n) fp32 device
+        tensors; the closest hits are recomputed."""
+        torch = _torch()
+        r, n = self._rays(rays)
+        m = self.tile.n_leaf
+        if grad is None:
+            grad = torch.zeros((m, m), dtype=torch.float64, device="cuda")
+        elif not (grad.dtype == torch.float64 and tuple(grad.shape) == (m, m) and grad.is_contiguous()):
+            raise DmapError(f"grad must be a contiguous {m} x {m} float64 device tensor")
+
+        def vec(x, shape):
+            if x is None:
+                return None
+            if not (isinstance(x, torch.Tensor) and x.is_cuda and x.dtype == torch.float32
+                    and tuple(x.shape) == shape and x.is_contiguous()):
+                raise DmapError(f"upstream gradients must be contiguous float32 device tensors of shape {shape}")
+            return x
+
+        lt = vec(dl_dt, (n,))
+        ln = vec(dl_dn, (3, n))
+        ptr = lambda x: ctypes.c_void_p(x.data_ptr()) if x is not None else None  # noqa: E731
+        opt = (options or TraversalOptions())._c()
+        _check(_lib.dmap_intersect_adjoint(self._h, ctypes.byref(r), n, ptr(lt),
+                                           ptr(ln[0]) if ln is not None else None,
+                                           ptr(ln[1]) if ln is not None else None,
+                                           ptr(ln[2]) if ln is not None else None, ctypes.c_void_p(grad.data_ptr()),
+                                           ctypes.byref(opt), _stream_ptr(stream)), "dmap_intersect_adjoint")
+        return grad
 
     def occluded(self, rays, options=None, prim_mask=-1, stream=None):
         torch = _torch()
@@ -351,3 +411,50 @@ class Server:
             _lib.dmap_server_destroy(self._h)
             self._h = None
 
+
+def displaced_hits(texels, tile, obj, rays, options=None):
+    """Differentiable closest hit: (t (n,), ng (3, n)) of the rays against
+    `obj` with the tile's texels set to `texels` ((n_leaf, n_leaf) float32
+    device tensor). Backward runs the adjoint kernel, which recomputes the
+    hits; misses (t = inf) carry no gradient."""
+    return _DisplacedHits.apply(texels, tile, obj, rays, options)
+
+
+def _hits_fn():
+    import torch
+
+    class DisplacedHitsFn(torch.autograd.Function):
+        @staticmethod
+        def forward(ctx, texels, tile, obj, rays, options):
+            tile.set_texels(texels)
+            obj.refresh()
+            res = obj.intersect(rays, options=options)
+            ctx.save_for_backward(texels, rays)
+            ctx.tile, ctx.obj, ctx.options = tile, obj, options
+            return res["t"].clone(), res["ng"].clone()
+
+        @staticmethod
+        def backward(ctx, g_t, g_n):
+            texels, rays = ctx.saved_tensors
+            tile, obj = ctx.tile, ctx.obj
+            # The tile may have been set to other texels since the forward pass.
+            tile.set_texels(texels)
+            obj.refresh()
+            lt = g_t.contiguous().to(torch.float32) if g_t is not None else None
+            ln = g_n.contiguous().to(torch.float32) if g_n is not None else None
+            grad = obj.adjoint(rays, lt, ln, options=ctx.options)
+            return grad.to(texels.dtype), None, None, None, None
+
+    return DisplacedHitsFn
+
+
+class _LazyFn:
+    _fn = None
+
+    def apply(self, *args):
+        if _LazyFn._fn is None:
+            _LazyFn._fn = _hits_fn()
+        return _LazyFn._fn.apply(*args)
+
+
+_DisplacedHits = _LazyFn()

@@ -427,6 +427,17 @@ static int tile_build(dmap_tile *t, hipStream_t stream)
     return DMAP_OK;
 }
 
+// The root cell's [h_min, h_max], served above the root; synchronizes `stream`.
+static int read_root(dmap_tile *t, hipStream_t stream)
+{
+    float2 root;
+    HIP_TRY(hipMemcpyAsync(&root, t->mm, sizeof root, hipMemcpyDeviceToHost, stream));
+    HIP_TRY(hipStreamSynchronize(stream));
+    t->root_hmin = root.x;
+    t->root_hmax = root.y;
+    return DMAP_OK;
+}
+
 int dmap_tile_create(dmap_ctx *ctx, const float *texels, int n, int texels_on_device, int repeat, double strength,
                      double offset, int build, void *stream_, dmap_tile **out)
 {
@@ -471,22 +482,28 @@ int dmap_tile_create(dmap_ctx *ctx, const float *texels, int n, int texels_on_de
         return fail(DMAP_ERR_HIP, "dmap_tile_create: %s", hipGetErrorString(e));
     }
     int rc = tile_build(t, stream);
+    if (rc == DMAP_OK)
+        rc = read_root(t, stream);
     if (rc != DMAP_OK) {
         tile_free(t);
         return rc;
     }
-    float2 root;
-    e = hipMemcpyAsync(&root, t->mm, sizeof root, hipMemcpyDeviceToHost, stream);
-    if (e == hipSuccess)
-        e = hipStreamSynchronize(stream);
-    if (e != hipSuccess) {
-        tile_free(t);
-        return fail(DMAP_ERR_HIP, "dmap_tile_create: %s", hipGetErrorString(e));
-    }
-    t->root_hmin = root.x;
-    t->root_hmax = root.y;
     *out = t;
     return DMAP_OK;
+}
+
+int dmap_tile_set_texels(dmap_tile *tile, const float *texels, int texels_on_device, void *stream_)
+{
+    if (!tile || !texels)
+        return fail(DMAP_ERR_INVALID, "dmap_tile_set_texels: NULL argument");
+    hipStream_t stream = (hipStream_t)stream_;
+    HIP_TRY(hipSetDevice(tile->ctx->device));
+    size_t nn = (size_t)tile->n * tile->n;
+    HIP_TRY(hipMemcpyAsync(tile->texels, texels, nn * 4,
+                           texels_on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, stream));
+    HIP_TRY(launch_close_tile(tile->texels, tile->nodes, tile->n, tile->repeat, stream));
+    int rc = tile_build(tile, stream);
+    return rc == DMAP_OK ? read_root(tile, stream) : rc;
 }
 
 int dmap_tile_rebuild(dmap_tile *tile, void *stream)
@@ -585,6 +602,69 @@ static void object_free(dmap_object *o)
     delete o;
 }
 
+// The triangles' height ranges over the tile's current pyramid, their
+// boxes, the top-level BVH and the object's device view (triangle_height_range,
+// triangle_aabb, node_bounds.cpp:218-269; ObjectIntersector::build_bvh,
+// displaced_intersector.cpp:540-576). Synchronizes the device.
+static int object_bounds(dmap_object *o, const std::vector<HostTri> &tris)
+{
+    const int n_tri = o->n_tri;
+    const dmap_tile *tile = o->tile;
+    hipError_t e;
+    float *h_range = nullptr;
+    e = hipMalloc(&h_range, sizeof(float) * 2 * n_tri);
+    if (e == hipSuccess)
+        e = hipMemcpy(o->tris, o->host.data(), sizeof(TriRecord) * n_tri, hipMemcpyHostToDevice);
+    if (e == hipSuccess)
+        e = launch_height_range(view_of(tile), o->tris, n_tri, h_range, nullptr);
+    o->h_range_host.resize(2 * (size_t)n_tri);
+    if (e == hipSuccess)
+        e = hipMemcpy(o->h_range_host.data(), h_range, sizeof(float) * 2 * n_tri, hipMemcpyDeviceToHost);
+    hipFree(h_range);
+    if (e != hipSuccess)
+        return fail(DMAP_ERR_HIP, "object bounds: %s", hipGetErrorString(e));
+    o->boxes_host.resize(6 * (size_t)n_tri);
+    for (int k = 0; k < n_tri; ++k) {
+        float lo = o->h_range_host[2 * k], hi = o->h_range_host[2 * k + 1];
+        if (!(lo <= hi))
+            return fail(DMAP_ERR_INVALID, "object bounds: triangle %d's footprint holds no cell", k);
+        triangle_box(tris[k], lo, hi, &o->boxes_host[6 * k]);
+        for (int q = 0; q < 3; ++q) {
+            o->host[k].aabb_lo[q] = o->boxes_host[6 * k + q];
+            o->host[k].aabb_hi[q] = o->boxes_host[6 * k + 3 + q];
+        }
+    }
+    std::vector<int> idx(n_tri);
+    for (int k = 0; k < n_tri; ++k)
+        idx[k] = k;
+    o->bvh_host.clear();
+    o->bvh_host.reserve(2 * (size_t)n_tri);
+    build_bvh_node(o->bvh_host, o->boxes_host, idx, 0, n_tri, -1);
+    o->const_normal = 1;
+    for (int k = 0; k < n_tri; ++k)
+        o->const_normal &= o->host[k].const_normal;
+    for (int a = 0; a < 3; ++a) {
+        o->bound_lo[a] = o->bvh_host[0].lo[a];
+        o->bound_hi[a] = o->bvh_host[0].hi[a];
+    }
+    o->view.pyr = view_of(tile);
+    o->view.tris = o->tris;
+    o->view.boxes = o->boxes;
+    o->view.n_tri = n_tri;
+    o->view.const_normal = o->const_normal;
+    o->view.bvh = o->bvh;
+    e = hipMemcpy(o->view_dev, &o->view, sizeof(ObjectDev), hipMemcpyHostToDevice);
+    if (e == hipSuccess)
+        e = hipMemcpy(o->bvh, o->bvh_host.data(), sizeof(BvhNode) * o->bvh_host.size(), hipMemcpyHostToDevice);
+    if (e == hipSuccess)
+        e = hipMemcpy(o->boxes, o->boxes_host.data(), sizeof(float) * 6 * n_tri, hipMemcpyHostToDevice);
+    if (e == hipSuccess)
+        e = hipMemcpy(o->tris, o->host.data(), sizeof(TriRecord) * n_tri, hipMemcpyHostToDevice);
+    if (e != hipSuccess)
+        return fail(DMAP_ERR_HIP, "object bounds: %s", hipGetErrorString(e));
+    return DMAP_OK;
+}
+
 int dmap_object_create(dmap_ctx *ctx, dmap_tile *tile, int n_tri, const double *positions, const double *normals,
                        const double *uvs, dmap_object **out)
 {
@@ -615,68 +695,36 @@ int dmap_object_create(dmap_ctx *ctx, dmap_tile *tile, int n_tri, const double *
         object_free(o);
         return fail(DMAP_ERR_MEMORY, "dmap_object_create: %s", hipGetErrorString(e));
     }
-    float *h_range = nullptr;
-    e = hipMalloc(&h_range, sizeof(float) * 2 * n_tri);
-    if (e == hipSuccess)
-        e = hipMemcpy(o->tris, o->host.data(), sizeof(TriRecord) * n_tri, hipMemcpyHostToDevice);
-    if (e == hipSuccess)
-        e = launch_height_range(view_of(tile), o->tris, n_tri, h_range, nullptr);
-    o->h_range_host.resize(2 * (size_t)n_tri);
-    if (e == hipSuccess)
-        e = hipMemcpy(o->h_range_host.data(), h_range, sizeof(float) * 2 * n_tri, hipMemcpyDeviceToHost);
-    hipFree(h_range);
-    if (e != hipSuccess) {
-        object_free(o);
-        return fail(DMAP_ERR_HIP, "dmap_object_create: %s", hipGetErrorString(e));
-    }
-    o->boxes_host.resize(6 * (size_t)n_tri);
-    for (int k = 0; k < n_tri; ++k) {
-        float lo = o->h_range_host[2 * k], hi = o->h_range_host[2 * k + 1];
-        if (!(lo <= hi)) {
-            object_free(o);
-            return fail(DMAP_ERR_INVALID, "dmap_object_create: triangle %d's footprint holds no cell", k);
-        }
-        triangle_box(tris[k], lo, hi, &o->boxes_host[6 * k]);
-        for (int q = 0; q < 3; ++q) {
-            o->host[k].aabb_lo[q] = o->boxes_host[6 * k + q];
-            o->host[k].aabb_hi[q] = o->boxes_host[6 * k + 3 + q];
-        }
-    }
-    std::vector<int> idx(n_tri);
-    for (int k = 0; k < n_tri; ++k)
-        idx[k] = k;
-    o->bvh_host.reserve(2 * (size_t)n_tri);
-    build_bvh_node(o->bvh_host, o->boxes_host, idx, 0, n_tri, -1);
-    o->const_normal = 1;
-    for (int k = 0; k < n_tri; ++k)
-        o->const_normal &= o->host[k].const_normal;
-    for (int a = 0; a < 3; ++a) {
-        o->bound_lo[a] = o->bvh_host[0].lo[a];
-        o->bound_hi[a] = o->bvh_host[0].hi[a];
-    }
-    o->view.pyr = view_of(tile);
-    o->view.tris = o->tris;
-    o->view.boxes = o->boxes;
-    o->view.n_tri = n_tri;
-    o->view.const_normal = o->const_normal;
-    e = hipMalloc(&o->bvh, sizeof(BvhNode) * o->bvh_host.size());
-    o->view.bvh = o->bvh;
+    o->positions.assign(positions, positions + 9 * (size_t)n_tri);
+    o->normals.assign(normals, normals + 9 * (size_t)n_tri);
+    o->uvs.assign(uvs, uvs + 6 * (size_t)n_tri);
+    e = hipMalloc(&o->bvh, sizeof(BvhNode) * (2 * (size_t)n_tri - 1));
     if (e == hipSuccess)
         e = hipMalloc(&o->view_dev, sizeof(ObjectDev));
-    if (e == hipSuccess)
-        e = hipMemcpy(o->view_dev, &o->view, sizeof(ObjectDev), hipMemcpyHostToDevice);
-    if (e == hipSuccess)
-        e = hipMemcpy(o->bvh, o->bvh_host.data(), sizeof(BvhNode) * o->bvh_host.size(), hipMemcpyHostToDevice);
-    if (e == hipSuccess)
-        e = hipMemcpy(o->boxes, o->boxes_host.data(), sizeof(float) * 6 * n_tri, hipMemcpyHostToDevice);
-    if (e == hipSuccess)
-        e = hipMemcpy(o->tris, o->host.data(), sizeof(TriRecord) * n_tri, hipMemcpyHostToDevice);
     if (e != hipSuccess) {
         object_free(o);
-        return fail(DMAP_ERR_HIP, "dmap_object_create: %s", hipGetErrorString(e));
+        return fail(DMAP_ERR_MEMORY, "dmap_object_create: %s", hipGetErrorString(e));
+    }
+    int rc = object_bounds(o, tris);
+    if (rc != DMAP_OK) {
+        object_free(o);
+        return rc;
     }
     *out = o;
     return DMAP_OK;
+}
+
+int dmap_object_refresh(dmap_object *obj)
+{
+    if (!obj)
+        return fail(DMAP_ERR_INVALID, "dmap_object_refresh: NULL object");
+    HIP_TRY(hipSetDevice(obj->ctx->device));
+    HIP_TRY(hipDeviceSynchronize());
+    std::vector<HostTri> tris(obj->n_tri);
+    for (int k = 0; k < obj->n_tri; ++k)
+        make_tri(obj->positions.data() + 9 * (size_t)k, obj->normals.data() + 9 * (size_t)k,
+                 obj->uvs.data() + 6 * (size_t)k, tris[k]);
+    return object_bounds(obj, tris);
 }
 
 int dmap_object_triangle_info(const dmap_object *obj, int k, int *root_level, int *n_roots, int32_t *roots,
@@ -790,6 +838,54 @@ int dmap_intersect_closest(const dmap_object *obj, const dmap_rays *rays, int64_
     if (!hits || !hits->t || !hits->s || !hits->tt || !hits->prim || !hits->texel_i || !hits->texel_j)
         return fail(DMAP_ERR_INVALID, "dmap_intersect_closest: hits t, s, tt, prim, texel_i and texel_j are required");
     return trace(obj, rays, n, hits, nullptr, prim_mask, options, stats, stream);
+}
+
+int dmap_intersect_adjoint(const dmap_object *obj, const dmap_rays *rays, int64_t n, const float *dl_dt,
+                           const float *dl_dnx, const float *dl_dny, const float *dl_dnz, double *grad_texels,
+                           const dmap_options *options, void *stream)
+{
+    if (!obj || !rays || !grad_texels)
+        return fail(DMAP_ERR_INVALID, "dmap_intersect_adjoint: NULL object, rays or gradient");
+    bool any_n = dl_dnx || dl_dny || dl_dnz, all_n = dl_dnx && dl_dny && dl_dnz;
+    if (any_n != all_n)
+        return fail(DMAP_ERR_INVALID, "dmap_intersect_adjoint: give all three normal gradients or none");
+    dmap_options o;
+    int rc = check_options(options, o);
+    if (rc != DMAP_OK)
+        return rc;
+    if (o.leaf != 0)
+        return fail(DMAP_ERR_INVALID, "dmap_intersect_adjoint: the adjoint follows the Newton leaf only");
+    if (n == 0)
+        return DMAP_OK;
+    if (n < 0 || n > 0x7fffffffLL)
+        return fail(DMAP_ERR_INVALID, "dmap_intersect_adjoint: bad ray count %lld", (long long)n);
+    HIP_TRY(hipSetDevice(obj->ctx->device));
+    hipStream_t s = (hipStream_t)stream;
+    // The closest hits are recomputed, not kept from the forward pass.
+    void *ws = nullptr;
+    HIP_TRY(hipMallocAsync(&ws, 44 * (size_t)n, s));
+    float *f = (float *)ws;
+    int32_t *ip = (int32_t *)(f + 8 * n);
+    dmap_hits h{f, f + n, f + 2 * n, f + 3 * n, f + 4 * n, f + 5 * n, f + 6 * n, f + 7 * n, ip, ip + n, ip + 2 * n};
+    rc = trace(obj, rays, n, &h, nullptr, -1, &o, nullptr, stream);
+    if (rc == DMAP_OK) {
+        AdjointParams ap;
+        std::memset(&ap, 0, sizeof ap);
+        ap.obj = obj->view;
+        ap.rays = RaysDev{rays->ox, rays->oy, rays->oz, rays->dx, rays->dy, rays->dz, rays->tmin, rays->tmax};
+        ap.hits = HitsDev{h.t, h.s, h.tt, h.bu, h.bv, h.nx, h.ny, h.nz, h.prim, h.texel_i, h.texel_j};
+        ap.dl_dt = dl_dt;
+        ap.dl_dn[0] = dl_dnx;
+        ap.dl_dn[1] = dl_dny;
+        ap.dl_dn[2] = dl_dnz;
+        ap.grad = grad_texels;
+        ap.n_rays = n;
+        hipError_t e = launch_adjoint(ap, s);
+        if (e != hipSuccess)
+            rc = fail(DMAP_ERR_HIP, "dmap_intersect_adjoint: %s", hipGetErrorString(e));
+    }
+    hipFreeAsync(ws, s);
+    return rc;
 }
 
 int dmap_intersect_any(const dmap_object *obj, const dmap_rays *rays, int64_t n, uint8_t *occluded, int prim_mask,

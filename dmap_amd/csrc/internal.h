@@ -47,6 +47,7 @@ struct dmap_object
     BvhNode *bvh = nullptr;
     ObjectDev view{};              // what a single-object launch passes
     ObjectDev *view_dev = nullptr; // the same, in device memory, for multi-request launches
+    std::vector<double> positions, normals, uvs; // the input triangles, for dmap_object_refresh
 };
 
 

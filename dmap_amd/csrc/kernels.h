@@ -107,6 +107,22 @@ size_t reorder_workspace_bytes(int64_t n);
 hipError_t reorder_rays(const RaysDev &rays, int64_t n, const float lo[3], const float hi[3], void *workspace,
                         size_t workspace_bytes, uint32_t **perm, hipStream_t stream);
 
+// Adjoint of the closest hit: from the hits of a closest-hit launch and the
+// upstream gradients dL/dt and dL/dng, adds dL/d(texel) into grad (n_leaf^2
+// doubles, row-major [j][i]) by the implicit-function theorem at each hit's
+// double root.
+struct AdjointParams
+{
+    ObjectDev obj;
+    RaysDev rays;
+    HitsDev hits;
+    const float *dl_dt;    // NULL: no depth term
+    const float *dl_dn[3]; // NULL: no normal term
+    double *grad;
+    int64_t n_rays;
+};
+hipError_t launch_adjoint(const AdjointParams &p, hipStream_t stream);
+
 // Closest hit / any hit over the object; precision 0 = fp32, 1 = fp64.
 // A closest-hit launch is followed by the finalize kernel, which turns the
 // raw record (t, prim, texel, texel-local root in s/tt) into the hit.

@@ -1501,6 +1501,142 @@ template <bool MULTI> __global__ void __launch_bounds__(256) k_finalize(TracePar
 }
 
 
+// ------------------------------------------------------------------ adjoint
+//
+// No counterpart in the reference. A hit is a root of F(a, b) = 0 with
+// F = ((S - o) . d1, (S - o) . d2) over the texel (texel-local a, b), where
+// S = P0 + a E1 + b E2 + h N, h = strength * bilinear(v00, v10, v01, v11)
+// + offset and N the (possibly interpolated) unit normal. For a corner value
+// v_c the implicit-function theorem gives (da, db) = -J^-1 dF/dv_c with
+// dF/dv_c = strength w_c (d1 . N, d2 . N); t = (S - o) . dn / |d| and
+// ng = unit(Sa x Sb) then move through S, Sa, Sb. The kernel forms the
+// vector-Jacobian product per ray: with lt = dL/dt, u = (I - ng ng^T)
+// dL/dng / |Sa x Sb|, A = Sb x u, B = u x Sa,
+//   mu = (lt dn . Sa / |d| + A . Saa + B . Sab, lt dn . Sb / |d| + A . Sab + B . Sbb),
+//   eta = J^-T mu,
+//   dL/dv_c = strength [lt w_c dn . N / |d| + A . (w_c,a N + w_c Na)
+//             + B . (w_c,b N + w_c Nb) - w_c (eta1 d1 . N + eta2 d2 . N)],
+// where Saa = 2 h_a Na + h Naa, Sab = h_ab N + h_a Nb + h_b Na + h Nab,
+// Sbb = 2 h_b Nb + h Nbb and, for N = M / |M| with M affine in (a, b),
+// Naa = -(2 Na (N . Mu) + N (Na . Mu)) / |M| (Nab, Nbb alike).
+// The root is the finalized hit's (s, t) brought back to the texel and
+// polished in double, so the gradient is the double solve's.
+
+__device__ inline int texel_of(int k, int n, bool repeat)
+{
+    // node k of the closed grid holds texel k mod n (repeat) or min(k, n - 1)
+    return repeat ? (k & (n - 1)) : min(max(k, 0), n - 1);
+}
+
+__global__ void __launch_bounds__(256) k_adjoint(AdjointParams p)
+{
+    const int64_t rid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (rid >= p.n_rays)
+        return;
+    const HitsDev &H = p.hits;
+    const int tri = H.prim[rid];
+    if (tri < 0)
+        return;
+    double lt = p.dl_dt ? (double)p.dl_dt[rid] : 0.0;
+    double ln[3] = {0.0, 0.0, 0.0};
+    if (p.dl_dn[0])
+        for (int q = 0; q < 3; ++q)
+            ln[q] = (double)p.dl_dn[q][rid];
+    if (!isfinite(lt) || !isfinite(ln[0]) || !isfinite(ln[1]) || !isfinite(ln[2]))
+        return;
+    if (lt == 0.0 && ln[0] == 0.0 && ln[1] == 0.0 && ln[2] == 0.0)
+        return;
+    const PyramidView &P = p.obj.pyr;
+    const TriRecord &T = p.obj.tris[tri];
+    const RaysDev &RY = p.rays;
+    const double o[3] = {RY.ox[rid], RY.oy[rid], RY.oz[rid]};
+    const double d[3] = {RY.dx[rid], RY.dy[rid], RY.dz[rid]};
+    const int i = H.texel_i[rid], j = H.texel_j[rid];
+    Leaf<double> L;
+    leaf_setup<double>(P, T, o, d, i, j, L);
+    double a = fmin(fmax(((double)H.s[rid] - L.slo) / L.w, 0.0), 1.0);
+    double b = fmin(fmax(((double)H.tt[rid] - L.tlo) / L.w, 0.0), 1.0);
+    polish_root(T, L, a, b);
+    polish_root(T, L, a, b);
+    // The surface and its derivatives at the root.
+    const double h = ((L.h00 + L.ha * a) + L.hb * b) + L.tw * a * b;
+    const double ha = L.ha + L.tw * b, hb = L.hb + L.tw * a, hab = L.tw;
+    double N[3], Na[3] = {0, 0, 0}, Nb[3] = {0, 0, 0}, Naa[3] = {0, 0, 0}, Nab[3] = {0, 0, 0}, Nbb[3] = {0, 0, 0};
+    if (T.const_normal) {
+        for (int q = 0; q < 3; ++q)
+            N[q] = L.N[q];
+    } else {
+        normal_at(L, a, b, N, Na, Nb);
+        double M[3];
+        for (int q = 0; q < 3; ++q)
+            M[q] = L.M0[q] + a * L.wMu[q] + b * L.wMv[q];
+        const double len = sqrt(M[0] * M[0] + M[1] * M[1] + M[2] * M[2]);
+        const double nu = N[0] * L.wMu[0] + N[1] * L.wMu[1] + N[2] * L.wMu[2];
+        const double nv = N[0] * L.wMv[0] + N[1] * L.wMv[1] + N[2] * L.wMv[2];
+        const double au = Na[0] * L.wMu[0] + Na[1] * L.wMu[1] + Na[2] * L.wMu[2];
+        const double bu = Nb[0] * L.wMu[0] + Nb[1] * L.wMu[1] + Nb[2] * L.wMu[2];
+        const double bv = Nb[0] * L.wMv[0] + Nb[1] * L.wMv[1] + Nb[2] * L.wMv[2];
+        for (int q = 0; q < 3; ++q) {
+            Naa[q] = -(2.0 * nu * Na[q] + au * N[q]) / len;
+            Nab[q] = -(nu * Nb[q] + nv * Na[q] + bu * N[q]) / len;
+            Nbb[q] = -(2.0 * nv * Nb[q] + bv * N[q]) / len;
+        }
+    }
+    double Sa[3], Sb[3], Saa[3], Sab[3], Sbb[3];
+    for (int q = 0; q < 3; ++q) {
+        Sa[q] = L.E1[q] + ha * N[q] + h * Na[q];
+        Sb[q] = L.E2[q] + hb * N[q] + h * Nb[q];
+        Saa[q] = 2.0 * ha * Na[q] + h * Naa[q];
+        Sab[q] = hab * N[q] + ha * Nb[q] + hb * Na[q] + h * Nab[q];
+        Sbb[q] = 2.0 * hb * Nb[q] + h * Nbb[q];
+    }
+    auto dot3 = [](const double *x, const double *y) { return x[0] * y[0] + x[1] * y[1] + x[2] * y[2]; };
+    const double J00 = dot3(L.d1, Sa), J01 = dot3(L.d1, Sb), J10 = dot3(L.d2, Sa), J11 = dot3(L.d2, Sb);
+    const double det = J00 * J11 - J01 * J10;
+    if (det == 0.0)
+        return;
+    // The normal term's vectors.
+    double nr[3] = {Sa[1] * Sb[2] - Sa[2] * Sb[1], Sa[2] * Sb[0] - Sa[0] * Sb[2], Sa[0] * Sb[1] - Sa[1] * Sb[0]};
+    const double nlen = sqrt(dot3(nr, nr));
+    double u[3] = {0, 0, 0}, A[3] = {0, 0, 0}, B[3] = {0, 0, 0};
+    if (nlen > 0.0) {
+        double ng[3] = {nr[0] / nlen, nr[1] / nlen, nr[2] / nlen};
+        const double gl = dot3(ng, ln);
+        for (int q = 0; q < 3; ++q)
+            u[q] = (ln[q] - ng[q] * gl) / nlen;
+        A[0] = Sb[1] * u[2] - Sb[2] * u[1];
+        A[1] = Sb[2] * u[0] - Sb[0] * u[2];
+        A[2] = Sb[0] * u[1] - Sb[1] * u[0];
+        B[0] = u[1] * Sa[2] - u[2] * Sa[1];
+        B[1] = u[2] * Sa[0] - u[0] * Sa[2];
+        B[2] = u[0] * Sa[1] - u[1] * Sa[0];
+    }
+    const double ltd = lt / L.d_len;
+    const double mua = ltd * dot3(L.dn, Sa) + dot3(A, Saa) + dot3(B, Sab);
+    const double mub = ltd * dot3(L.dn, Sb) + dot3(A, Sab) + dot3(B, Sbb);
+    const double eta1 = (J11 * mua - J10 * mub) / det, eta2 = (-J01 * mua + J00 * mub) / det;
+    const double dnN = dot3(L.dn, N), d1N = dot3(L.d1, N), d2N = dot3(L.d2, N);
+    const double AN = dot3(A, N), BN = dot3(B, N), ANa = dot3(A, Na), BNb = dot3(B, Nb);
+    const double scale = P.scale_d;
+    const int n = P.n_leaf;
+    for (int c = 0; c < 4; ++c) {
+        const int di = c & 1, dj = c >> 1;
+        const double wu = di ? a : 1.0 - a, wv = dj ? b : 1.0 - b;
+        const double w = wu * wv, wa = (di ? 1.0 : -1.0) * wv, wb = wu * (dj ? 1.0 : -1.0);
+        const double g = scale * (ltd * w * dnN + AN * wa + w * ANa + BN * wb + w * BNb - w * (eta1 * d1N + eta2 * d2N));
+        const int ti = texel_of(i + di, n, P.repeat), tj = texel_of(j + dj, n, P.repeat);
+        unsafeAtomicAdd(p.grad + (size_t)tj * n + ti, g);
+    }
+}
+
+hipError_t launch_adjoint(const AdjointParams &p, hipStream_t stream)
+{
+    if (p.n_rays <= 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(k_adjoint, dim3((unsigned)((p.n_rays + 255) / 256)), dim3(256), 0, stream, p);
+    return hipGetLastError();
+}
+
 // ------------------------------------------------------- height range
 
 // classify_square in double (uv_clip.cpp:27-44): 0 outside, 1 inside, 2 straddle.

@@ -20,6 +20,12 @@ the CPU oracle (both read the same fp32 arrays):
                     uniform surface points, origins lifted 1e-4 along the
                     surface normal.
 
+  C4 "adjoint":     one unit quad, 1024^2 map of 4-octave fBm at amplitude
+                    0.05, the C1 camera at 512 x 512; targets are rendered
+                    from the same map plus 16 seeded Gaussian blobs of
+                    sigma 8 texels and height amplitude 0.01 (0.2 in texel
+                    units), wrapped with the tile.
+
 A texel value v gives the height h = strength * v (offset 0): the maps are
 stored in units of their amplitude.
 """
@@ -226,3 +232,36 @@ def c2_incoherent(n=2048, width=2048, height=1024, n_bounce=None, nq=8, size=8.0
     return Workload("c2_incoherent", tex, 0.1, 0.0, True, tris, rays, o1.shape[0],
                     f"{nq*nq} quads on [0,{size:g}]^2, {n}^2 fBm(6)+ridged amp 0.1/0.03, "
                     f"{o1.shape[0]} primaries + {nb} cosine bounces")
+
+
+def gaussian_blobs(n, count, sigma, amplitude, seed):
+    """Sum of `count` Gaussian bumps (sigma in texels, peak `amplitude`) at
+    seeded centres and signs, periodic over the n x n tile. [j, i]."""
+    rng = np.random.default_rng(seed)
+    x = np.arange(n, dtype=np.float64)
+    out = np.zeros((n, n))
+    for _ in range(count):
+        cx, cy = rng.uniform(0, n, 2)
+        sign = 1.0 if rng.random() < 0.5 else -1.0
+        dx = np.abs(x - cx)
+        dx = np.minimum(dx, n - dx)
+        dy = np.abs(x - cy)
+        dy = np.minimum(dy, n - dy)
+        out += sign * amplitude * np.exp(-(dy[:, None] ** 2 + dx[None, :] ** 2) / (2 * sigma * sigma))
+    return out
+
+
+class AdjointWorkload(Workload):
+    def __init__(self, texels, target_texels, strength, triangles, rays, description):
+        super().__init__("c4_adjoint", texels, strength, 0.0, True, triangles, rays, rays.shape[1], description)
+        self.target_texels = target_texels.astype(np.float32)
+
+
+def c4_adjoint(n=1024, width=512, height=512, blobs=16):
+    strength = 0.05
+    tex = fbm(n, 4, max(4, n // 64), NOISE_SEED).astype(np.float32)
+    target = tex + gaussian_blobs(n, blobs, 8.0 * n / 1024, 0.01 / strength, NOISE_SEED + 5)
+    tris = quad_plane(1, 1.0)
+    o, d = pinhole_rays((0.5, -0.8, 0.9), (0.5, 0.5, 0.0), 60.0, width, height)
+    return AdjointWorkload(tex, target, strength, tris, pack_rays(o, d),
+                           f"1 quad, {n}^2 fBm(4) amp 0.05, {width}x{height} primaries, targets +{blobs} blobs")

@@ -14,6 +14,8 @@
  *                         and, with prim_mask = k, DisplacedIntersector::intersect
  *                         (displaced_intersector.h:137-138) on triangle k alone
  *   dmap_intersect_any    the occluded callback (displaced_geometry.cpp:79-106)
+ * The adjoint, tile texel updates and the serving loop have no counterpart
+ * in the reference (see their declarations).
  *
  * Conventions (as the reference's, displaced_intersector.h:107-114):
  *   rays are in object space, d need not be unit length, t is in units of
@@ -100,6 +102,11 @@ int dmap_tile_create(dmap_ctx *ctx, const float *texels, int n, int texels_on_de
                      double offset, int build, void *stream, dmap_tile **out);
 /* Re-runs the pyramid build from the resident node grid (for timing). */
 int dmap_tile_rebuild(dmap_tile *tile, void *stream);
+/* New texel values for the tile (same size): closes the grid and rebuilds
+ * the pyramid on `stream`, and waits. Objects over the tile then need
+ * dmap_object_refresh (no counterpart in the reference, whose assets are
+ * immutable; the differentiable path updates texels between passes). */
+int dmap_tile_set_texels(dmap_tile *tile, const float *texels, int texels_on_device, void *stream);
 int dmap_tile_info(const dmap_tile *tile, int *n_leaf, int *n_levels, int64_t *device_bytes);
 /* The eight channels of one level (TaylorPyramid::levels, taylor_pyramid.h:51-61)
  * to host memory: out holds 8 x m x m floats, channel-major, each channel
@@ -121,6 +128,9 @@ int dmap_object_create(dmap_ctx *ctx, dmap_tile *tile, int n_tri, const double *
  * range over the footprint. */
 int dmap_object_triangle_info(const dmap_object *obj, int k, int *root_level, int *n_roots, int32_t *roots,
                               float *aabb, float *h_range);
+/* Recomputes the triangles' height ranges, AABBs and BVH from the tile's
+ * current pyramid (after dmap_tile_set_texels). Synchronizes the device. */
+int dmap_object_refresh(dmap_object *obj);
 void dmap_object_destroy(dmap_object *obj);
 
 /* Closest hit of n rays. prim_mask -1 tests every triangle, k >= 0 only
@@ -132,6 +142,18 @@ int dmap_intersect_closest(const dmap_object *obj, const dmap_rays *rays, int64_
 /* Any hit: occluded[r] = 1 when the ray meets the surface in (tmin, tmax). */
 int dmap_intersect_any(const dmap_object *obj, const dmap_rays *rays, int64_t n, uint8_t *occluded, int prim_mask,
                        const dmap_options *options, void *stream);
+
+/* Adjoint of the closest hit (no counterpart in the reference): for the
+ * loss gradients dl_dt[r] = dL/dt_r and (dl_dnx, dl_dny, dl_dnz)[r] =
+ * dL/dn_r (device fp32 arrays; the normal ones all NULL for a depth-only
+ * loss), ADDS dL/d(texel) into grad_texels (device, n_leaf x n_leaf
+ * doubles, row-major [j][i]). The closest hits are recomputed (not kept
+ * from a forward call) and the gradient follows the implicit-function
+ * theorem at each hit's double root; misses contribute nothing. Newton leaf
+ * only; options as dmap_intersect_closest's. */
+int dmap_intersect_adjoint(const dmap_object *obj, const dmap_rays *rays, int64_t n, const float *dl_dt,
+                           const float *dl_dnx, const float *dl_dny, const float *dl_dnz, double *grad_texels,
+                           const dmap_options *options, void *stream);
 
 /* Serving (continuous batching; no counterpart in the reference, which
  * traces one ray per call). Requests against resident objects are queued,

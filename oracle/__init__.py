@@ -91,6 +91,7 @@ def _declare(L):
     L.orc_field_eval.argtypes = [_VP, c.c_int64, _D, _D, _D]
     L.orc_classify_square.argtypes = [_D, c.c_double, c.c_double, c.c_double]
     L.orc_surface_point.argtypes = [_VP, c.c_int, c.c_double, c.c_double, _D]
+    L.orc_scene_adjoint.argtypes = [_VP, c.c_int64, _D, _D, _D, _D, _D, _D, c.c_int, _D, _D, _D]
 
 
 def _p(a, t=_D):
@@ -199,6 +200,24 @@ class Scene:
         lib().orc_scene_occluded(self.handle, n, _p(o), _p(d), _p(tmin), _p(tmax), BOUND[bound],
                                  int(slab_max_level), LEAF[leaf], int(threads), _p(occ, _U8))
         return occ.astype(bool)
+
+    def adjoint(self, o, d, dl_dt, dl_dn=None, tmin=None, tmax=None, threads=0):
+        """Gradient of sum_r dl_dt[r] t_r + dl_dn[r] . ng_r with respect to
+        the closed grid's node values (nodes x nodes), by the implicit-function
+        theorem at each closest hit's double root; also that root's t and ng."""
+        o, d = _f64(o).reshape(-1, 3), _f64(d).reshape(-1, 3)
+        n = o.shape[0]
+        tmin = np.zeros(n) if tmin is None else _f64(np.broadcast_to(tmin, (n,)))
+        tmax = np.full(n, np.inf) if tmax is None else _f64(np.broadcast_to(tmax, (n,)))
+        lt = _f64(np.broadcast_to(dl_dt, (n,)))
+        ln = None if dl_dn is None else _f64(np.broadcast_to(dl_dn, (n, 3)))
+        m = self.nodes.shape[0]
+        grad = np.zeros((m, m))
+        t = np.empty(n)
+        ng = np.empty((n, 3))
+        lib().orc_scene_adjoint(self.handle, n, _p(o), _p(d), _p(tmin), _p(tmax), _p(lt),
+                                _p(ln) if ln is not None else None, int(threads), _p(grad), _p(t), _p(ng))
+        return grad, t, ng
 
     @staticmethod
     def _hit(ok, h):

@@ -1427,6 +1427,144 @@ void orc_scene_intersect(void *scene, int64_t n, const double *o, const double *
     });
 }
 
+// ------------------------------------------------------------------ adjoint
+//
+// No counterpart in the reference. The closest hit is a root of
+// S(s, t) = o + tau d with S = P + h N (displaced_position,
+// displaced_surface.cpp:162-166) and h bilinear in the four nodes of the cell
+// holding (s, t) (HeightGrid::h, displaced_surface.cpp:12-39). For a node
+// value v the implicit-function theorem on the 3x3 system
+// [S_s | S_t | -d] (ds, dt, dtau) = -dS/dv gives dtau/dv, and the normal
+// ng = unit(S_s x S_t) (displaced_intersector.cpp:357-366) moves through
+// dS_s = dS_s/dv + S_ss ds + S_st dt and dS_t alike. Everything is in tile
+// coordinates (s, t) with the grid's own bilinear weights.
+
+namespace
+{
+struct Geo
+{
+    V3 S, Ss, St, Sss, Sst, Stt, N, Nu, Nv;
+    int ci = 0, cj = 0;
+    double fs = 0.0, ft = 0.0;
+};
+
+// S and its first and second (s, t) derivatives. N = M / |M| with M affine
+// in (s, t): N_u = (M_u - N (N . M_u)) / |M| (normal_frame_at,
+// displaced_surface.cpp:99-108), and differentiating once more
+// N_uu = -(2 N_u (N . M_u) + N (N_u . M_u)) / |M|, the mixed and vv terms alike.
+Geo surface_geometry(const Tri &tri, const Grid &g, double s, double t)
+{
+    Geo q;
+    g.cell(s, t, q.ci, q.cj, q.fs, q.ft);
+    const double n1 = g.W - 1, fs = q.fs, ft = q.ft;
+    const double v00 = g.value(q.cj, q.ci), v10 = g.value(q.cj, q.ci + 1), v01 = g.value(q.cj + 1, q.ci),
+                 v11 = g.value(q.cj + 1, q.ci + 1);
+    const double h = g.scale * ((1 - fs) * (1 - ft) * v00 + fs * (1 - ft) * v10 + (1 - fs) * ft * v01 + fs * ft * v11) +
+                     g.offset;
+    const double hs = g.scale * n1 * ((1 - ft) * (v10 - v00) + ft * (v11 - v01));
+    const double ht = g.scale * n1 * ((1 - fs) * (v01 - v00) + fs * (v11 - v10));
+    const double hst = g.scale * n1 * n1 * (v11 - v10 - v01 + v00);
+    const V3 M = tri.M(s, t);
+    const double len = norm(M);
+    const V3 N = M / len;
+    const double nu = dot(N, tri.Mu), nv = dot(N, tri.Mv);
+    const V3 Nu = (tri.Mu - nu * N) / len, Nv = (tri.Mv - nv * N) / len;
+    const V3 Nuu = -1.0 / len * (2.0 * nu * Nu + dot(Nu, tri.Mu) * N);
+    const V3 Nuv = -1.0 / len * (nu * Nv + nv * Nu + dot(Nv, tri.Mu) * N);
+    const V3 Nvv = -1.0 / len * (2.0 * nv * Nv + dot(Nv, tri.Mv) * N);
+    q.N = N;
+    q.Nu = Nu;
+    q.Nv = Nv;
+    q.S = tri.P(s, t) + h * N;
+    q.Ss = tri.e1 + hs * N + h * Nu;
+    q.St = tri.e2 + ht * N + h * Nv;
+    q.Sss = 2.0 * hs * Nu + h * Nuu;
+    q.Sst = hst * N + hs * Nv + ht * Nu + h * Nuv;
+    q.Stt = 2.0 * ht * Nv + h * Nvv;
+    return q;
+}
+} // namespace
+
+void orc_scene_adjoint(void *scene, int64_t n, const double *o, const double *d, const double *tmin,
+                       const double *tmax, const double *dl_dt, const double *dl_dn, int nthreads, double *grad_nodes,
+                       double *t_out, double *ng_out)
+{
+    const Scene *sc = (const Scene *)scene;
+    const Grid &g = sc->grid;
+    Options opt{2, 1, 0};
+    std::vector<int64_t> node(4 * (size_t)n, -1);
+    std::vector<double> val(4 * (size_t)n, 0.0);
+    parallel_rays(n, nthreads, [&](int64_t r) {
+        const V3 O = v3(o[3 * r], o[3 * r + 1], o[3 * r + 2]), D = v3(d[3 * r], d[3 * r + 1], d[3 * r + 2]);
+        Hit hit;
+        int k = -1;
+        if (t_out)
+            t_out[r] = INFINITY;
+        if (ng_out)
+            ng_out[3 * r] = ng_out[3 * r + 1] = ng_out[3 * r + 2] = 0.0;
+        if (!sc->intersect(O, D, tmin[r], tmax[r], opt, hit, k, nullptr, false))
+            return;
+        const Tri &tri = sc->tris[k];
+        // Newton on F = ((S - o) . d1, (S - o) . d2) to the double root.
+        const V3 dn = normalized(D);
+        V3 d1, d2;
+        plane_basis(dn, d1, d2);
+        double s = hit.st.x, t = hit.st.y;
+        for (int it = 0; it < 8; ++it) {
+            Geo q = surface_geometry(tri, g, s, t);
+            V3 rel = q.S - O;
+            double F1 = dot(rel, d1), F2 = dot(rel, d2);
+            double J00 = dot(d1, q.Ss), J01 = dot(d1, q.St), J10 = dot(d2, q.Ss), J11 = dot(d2, q.St);
+            double det = J00 * J11 - J01 * J10;
+            if (det == 0.0)
+                break;
+            double ds = (J11 * F1 - J01 * F2) / det, dt = (-J10 * F1 + J00 * F2) / det;
+            s -= ds;
+            t -= dt;
+            if (std::abs(ds) + std::abs(dt) < 1e-17)
+                break;
+        }
+        const Geo q = surface_geometry(tri, g, s, t);
+        const double tau = dot(q.S - O, D) / dot(D, D);
+        const V3 nr = cross(q.Ss, q.St);
+        const double nlen = norm(nr);
+        const V3 ng = nr / nlen;
+        if (t_out)
+            t_out[r] = tau;
+        if (ng_out) {
+            ng_out[3 * r] = ng.x;
+            ng_out[3 * r + 1] = ng.y;
+            ng_out[3 * r + 2] = ng.z;
+        }
+        const double lt = dl_dt ? dl_dt[r] : 0.0;
+        const V3 ln = dl_dn ? v3(dl_dn[3 * r], dl_dn[3 * r + 1], dl_dn[3 * r + 2]) : V3{};
+        M3 A;
+        for (int row = 0; row < 3; ++row) {
+            const double c[3][3] = {{q.Ss.x, q.St.x, -D.x}, {q.Ss.y, q.St.y, -D.y}, {q.Ss.z, q.St.z, -D.z}};
+            for (int col = 0; col < 3; ++col)
+                A.m[row][col] = c[row][col];
+        }
+        const M3 Ainv = inverse(A);
+        const double n1 = g.W - 1;
+        for (int c = 0; c < 4; ++c) {
+            const int di = c & 1, dj = c >> 1;
+            const double wu = di ? q.fs : 1.0 - q.fs, wv = dj ? q.ft : 1.0 - q.ft;
+            const double w = wu * wv, ws = n1 * (di ? 1.0 : -1.0) * wv, wt = n1 * wu * (dj ? 1.0 : -1.0);
+            const V3 dS = g.scale * w * q.N;
+            const V3 dSs = g.scale * (ws * q.N + w * q.Nu), dSt = g.scale * (wt * q.N + w * q.Nv);
+            const V3 x = Ainv.mul(-dS); // (ds, dt, dtau)
+            const V3 DSs = dSs + x.x * q.Sss + x.y * q.Sst, DSt = dSt + x.x * q.Sst + x.y * q.Stt;
+            const V3 dnr = cross(DSs, q.St) + cross(q.Ss, DSt);
+            const V3 dng = (dnr - dot(ng, dnr) * ng) / nlen;
+            node[4 * r + c] = (int64_t)(q.cj + dj) * g.W + (q.ci + di);
+            val[4 * r + c] = lt * x.z + dot(ln, dng);
+        }
+    });
+    for (int64_t q = 0; q < 4 * n; ++q)
+        if (node[q] >= 0)
+            grad_nodes[node[q]] += val[q];
+}
+
 void orc_scene_occluded(void *scene, int64_t n, const double *o, const double *d, const double *tmin,
                         const double *tmax, int bound, int slab_max_level, int leaf, int nthreads, uint8_t *occluded)
 {

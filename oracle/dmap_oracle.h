@@ -111,6 +111,15 @@ int orc_classify_square(const double *t6, double cx, double cy, double s);
  * scale. */
 void orc_surface_point(void *scene, int k, double u, double v, double *out12);
 
+/* Adjoint of the closest hit (no counterpart in the reference): adds into
+ * grad_nodes (nodes x nodes, row-major [j][i]) the gradient of
+ * sum_r dl_dt[r] t_r + dl_dn[r] . ng_r with respect to the closed grid's node
+ * values, by the implicit-function theorem at each hit's double root. Also
+ * writes that root's t and ng (misses: inf and 0). dl_dn (n x 3) may be NULL. */
+void orc_scene_adjoint(void *scene, int64_t n, const double *o, const double *d, const double *tmin,
+                       const double *tmax, const double *dl_dt, const double *dl_dn, int nthreads, double *grad_nodes,
+                       double *t_out, double *ng_out);
+
 #ifdef __cplusplus
 }
 #endif
