@@ -30,6 +30,11 @@ FP32_PEAK_TFLOPS = 157.3
 BYTES_RAY_IO = 32 + 32
 BYTES_NODE = 32
 BYTES_LEAF = 16
+# HBM bytes the traversal kernel moves per ray, from rocprofv3 --pmc
+# FETCH_SIZE and WRITE_SIZE on this workload, each scaled by the same
+# counter's reading for a device copy of known size (bench.py
+# --calibrate-copy); profiles/k_trace_c2_incoherent_traffic.md.
+TRAFFIC_BYTES_PER_RAY = 1084.8
 # fp32 operations per unit of work, counted from the restated formulas
 # (node test with box + slab, sub-square bound, Newton iteration).
 FLOPS_NODE = 200
@@ -56,7 +61,13 @@ def main():
                     help="serving arrival rate as a fraction of the plain kernel's Mrays/s on the same rays")
     ap.add_argument("--serve-slots", type=int, default=4)
     ap.add_argument("--serve-step", type=int, default=1 << 18, help="rays per serving step")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint timing section")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="only run device-to-device copies of MIB MiB (the PMC byte-counter reference) and exit")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        calibrate_copy(args.calibrate_copy)
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus}); "
               "rays shard by index range with no exchange, see DESIGN.md", file=sys.stderr)
@@ -86,6 +97,8 @@ def main():
 
     starts = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
     ends = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
+    dmap_amd.kernel_timing_read()
+    dmap_amd.kernel_timing(True)
     torch.cuda.synchronize()
     wall0 = time.perf_counter()
     for k in range(args.steps):
@@ -94,13 +107,17 @@ def main():
         ends[k].record(stream)
     torch.cuda.synchronize()
     wall = time.perf_counter() - wall0
-    launch_ms = [s.elapsed_time(e) for s, e in zip(starts, ends)]
-    avg_ms = float(np.mean(launch_ms))
+    dmap_amd.kernel_timing(False)
+    # The traversal kernel alone: HIP events the library records around it
+    # on the launch's stream.
+    trace_ms_sum, launches = dmap_amd.kernel_timing_read()
+    avg_ms = trace_ms_sum / max(launches, 1)
+    call_ms = float(np.mean([s.elapsed_time(e) for s, e in zip(starts, ends)]))
     ms_per_step = wall * 1e3 / args.steps
     value = n / (ms_per_step * 1e-3) / 1e6
     t_gpu = hits["f"][0].cpu().numpy()
-    log(f"[bench] {args.steps} steps: {ms_per_step:.3f} ms/step (kernel {avg_ms:.3f} ms), {value:.1f} Mrays/s, "
-        f"hit fraction {np.isfinite(t_gpu).mean():.4f}")
+    log(f"[bench] {args.steps} steps: {ms_per_step:.3f} ms/step (call {call_ms:.3f} ms, traversal kernel "
+        f"{avg_ms:.3f} ms over {launches} launches), {value:.1f} Mrays/s, hit fraction {np.isfinite(t_gpu).mean():.4f}")
 
     # Tile build (kernel #1) timing, for the record.
     b0, b1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
@@ -139,8 +156,10 @@ def main():
         flops_ray = FLOPS_NODE * node + FLOPS_SUB * sub + FLOPS_NEWTON * its
         achieved = bytes_ray * n / (avg_ms * 1e-3) / 1e9
         roof = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
-                "kernel": "k_trace<float>", "bytes_per_ray": round(bytes_ray, 1)}
+                "frac": round(achieved / HBM_PEAK_GBS, 5),
+                "traffic": int(TRAFFIC_BYTES_PER_RAY * n) if TRAFFIC_BYTES_PER_RAY else None,
+                "kernel": "k_trace<float>", "kernel_ms": round(avg_ms, 4), "bytes_per_ray": round(bytes_ray, 1),
+                "rays_per_launch": n}
         extra["valu"] = {"achieved_tflops": round(flops_ray * n / (avg_ms * 1e-3) / 1e12, 3),
                          "peak_tflops": FP32_PEAK_TFLOPS,
                          "frac": round(flops_ray * n / (avg_ms * 1e-3) / 1e12 / FP32_PEAK_TFLOPS, 5),
@@ -153,12 +172,27 @@ def main():
         extra["sample_parity"] = {"hitmiss_flips": int((np.isfinite(gt) != np.isfinite(ref["t"])).sum()),
                                   "worst_t_rel": float(np.max(np.abs(gt[both] - ref["t"][both]) / ref["t"][both]))
                                   if both.any() else 0.0}
+        # One thread, best of 3 with a checksum of t (the reference's own
+        # method, test_traversal_cost.cpp:117-135), on every 64th ray.
+        idx1 = np.arange(0, n, 64)
+        best = np.inf
+        for _ in range(3):
+            c0 = time.perf_counter()
+            r1 = sc.intersect(o[::16], d[::16], wl.rays[6, idx1].astype(np.float64),
+                              wl.rays[7, idx1].astype(np.float64), threads=1)
+            best = min(best, time.perf_counter() - c0)
+        checksum = float(np.sum(r1["t"][np.isfinite(r1["t"])]))
         cpu = {"value": round(cpu_mrays, 4), "unit": "Mrays/s", "cores": threads, "kind": "port",
                "sample": f"every {args.cpu_sample}th ray of the same 4,194,304 ({len(idx)} rays), "
                          f"C++ double oracle, {threads} std::threads, {cpu_s:.2f}s",
-               "gpu_over_cpu": round(value / cpu_mrays, 1)}
+               "gpu_over_cpu": round(value / cpu_mrays, 1),
+               "single_thread": {"value": round(len(idx1) / best / 1e6, 4), "unit": "Mrays/s", "cores": 1,
+                                 "sample": f"every 64th ray ({len(idx1)} rays), best of 3",
+                                 "checksum_t": round(checksum, 6)}}
         log(f"[bench] cpu {cpu_mrays:.3f} Mrays/s; gpu/cpu {value / cpu_mrays:.1f}x")
 
+    if not args.no_adjoint:
+        extra["adjoint"] = adjoint(log)
     if args.serve_seconds > 0:
         extra["serving"] = serve(args, log)
 
@@ -174,6 +208,63 @@ def main():
                              "GB_s": round(build_bytes / (build_ms * 1e-3) / 1e9, 1)},
            **extra}
     print(json.dumps(out), flush=True)
+
+
+def calibrate_copy(mib, repeats=4):
+    """A plain device copy of known size: what FETCH_SIZE and WRITE_SIZE read
+    for it scales the counters of the traversal kernel to bytes."""
+    import torch
+    src = torch.ones(mib << 18, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(repeats):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes": mib << 20, "repeats": repeats}), flush=True)
+
+
+def adjoint(log, repeats=10):
+    """The differentiation workload (1024^2 fBm at amplitude 0.05 on one
+    quad, 512^2 primaries, L2 loss on depth and normal against blob-perturbed
+    targets): the forward closest hit and the adjoint pass (which recomputes
+    the hits, then scatters dL/d(texel)), timed with HIP events."""
+    import torch
+
+    import dmap_amd
+    from dmap_amd import workload as W
+
+    wl = W.c4_adjoint()
+    tile = dmap_amd.DisplacementTile(wl.texels, wl.strength)
+    obj = dmap_amd.ObjectIntersector(tile, wl.triangles)
+    tgt_obj = dmap_amd.ObjectIntersector(dmap_amd.DisplacementTile(wl.target_texels, wl.strength), wl.triangles)
+    rays = torch.from_numpy(wl.rays).cuda()
+    n = rays.shape[1]
+    tgt = tgt_obj.intersect(rays)
+    res = obj.intersect(rays)
+    mask = torch.isfinite(res["t"]) & torch.isfinite(tgt["t"])
+    lt = torch.where(mask, 2 * (res["t"] - tgt["t"]), torch.zeros_like(res["t"])).contiguous()
+    ln = torch.where(mask[None], 2 * (res["ng"] - tgt["ng"]), torch.zeros_like(res["ng"])).contiguous()
+    grad = torch.zeros((wl.texels.shape[0],) * 2, dtype=torch.float64, device="cuda")
+    hits = obj.alloc_hits(n)
+    stream = torch.cuda.current_stream()
+    obj.adjoint(rays, lt, ln, grad=grad)
+    torch.cuda.synchronize()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+    ev[0].record(stream)
+    for _ in range(repeats):
+        obj.intersect(rays, out=hits)
+    ev[1].record(stream)
+    for _ in range(repeats):
+        grad.zero_()
+        obj.adjoint(rays, lt, ln, grad=grad)
+    ev[2].record(stream)
+    torch.cuda.synchronize()
+    fwd_ms = ev[0].elapsed_time(ev[1]) / repeats
+    adj_ms = ev[1].elapsed_time(ev[2]) / repeats
+    log(f"[adjoint] forward {fwd_ms:.3f} ms, adjoint (with recomputed hits) {adj_ms:.3f} ms over {n} rays, "
+        f"{int(mask.sum())} contributing")
+    return {"workload": "1 quad, 1024^2 fBm(4) amp 0.05, 512x512 primaries, L2 on depth+normal vs 16-blob targets",
+            "rays": n, "contributing_rays": int(mask.sum()), "forward_ms": round(fwd_ms, 4),
+            "adjoint_ms": round(adj_ms, 4), "adjoint_mrays_s": round(n / (adj_ms * 1e-3) / 1e6, 2)}
 
 
 def serve(args, log):

@@ -39,7 +39,7 @@ EXPORTS = ("dmap_last_error", "dmap_version", "dmap_ctx_create", "dmap_ctx_destr
            "dmap_tile_rebuild", "dmap_tile_info", "dmap_tile_read_level", "dmap_tile_read_nodes",
            "dmap_tile_destroy", "dmap_object_create", "dmap_object_triangle_info", "dmap_object_destroy",
            "dmap_intersect_closest", "dmap_intersect_any", "dmap_tile_set_texels", "dmap_object_refresh",
-           "dmap_intersect_adjoint", "dmap_server_create", "dmap_server_submit", "dmap_server_poll",
+           "dmap_intersect_adjoint", "dmap_kernel_timing", "dmap_kernel_timing_read", "dmap_server_create", "dmap_server_submit", "dmap_server_poll",
            "dmap_server_hits", "dmap_server_copy_hits", "dmap_server_release", "dmap_server_idle",
            "dmap_server_destroy")
 
@@ -98,6 +98,8 @@ def load_library():
                                              c.POINTER(_Options), c.POINTER(_Stats), vp]
         L.dmap_intersect_any.argtypes = [vp, c.POINTER(_Rays), c.c_int64, vp, c.c_int, c.POINTER(_Options), vp]
         L.dmap_tile_set_texels.argtypes = [vp, vp, c.c_int, vp]
+        L.dmap_kernel_timing.argtypes = [c.c_int]
+        L.dmap_kernel_timing_read.argtypes = [c.POINTER(c.c_double), c.POINTER(c.c_int64)]
         L.dmap_object_refresh.argtypes = [vp]
         L.dmap_intersect_adjoint.argtypes = [vp, c.POINTER(_Rays), c.c_int64, vp, vp, vp, vp, vp,
                                              c.POINTER(_Options), vp]
@@ -410,6 +412,18 @@ class Server:
         if getattr(self, "_h", None) and self._h.value and _lib is not None:
             _lib.dmap_server_destroy(self._h)
             self._h = None
+
+
+def kernel_timing(enable):
+    """Record HIP events around every traversal kernel from now on (or stop)."""
+    _check(load_library().dmap_kernel_timing(1 if enable else 0), "dmap_kernel_timing")
+
+
+def kernel_timing_read():
+    """(summed traversal-kernel ms, launches) since the last read."""
+    ms, n = ctypes.c_double(), ctypes.c_int64()
+    _check(load_library().dmap_kernel_timing_read(ctypes.byref(ms), ctypes.byref(n)), "dmap_kernel_timing_read")
+    return ms.value, n.value
 
 
 def displaced_hits(texels, tile, obj, rays, options=None):

@@ -769,6 +769,12 @@ static int check_options(const dmap_options *opt, dmap_options &o)
     return DMAP_OK;
 }
 
+// Kernel timing: when on, each trace records a pair of events around the
+// traversal kernel; dmap_kernel_timing_read sums their durations.
+static std::mutex g_timing_mu;
+static bool g_timing = false;
+static std::vector<std::pair<hipEvent_t, hipEvent_t>> g_timing_events;
+
 static int trace(const dmap_object *obj, const dmap_rays *rays, int64_t n, const dmap_hits *hits, uint8_t *occluded,
                  int prim_mask, const dmap_options *options, const dmap_ray_stats *stats, void *stream)
 {
@@ -823,11 +829,51 @@ static int trace(const dmap_object *obj, const dmap_rays *rays, int64_t n, const
         }
         p.perm = perm;
     }
-    hipError_t e = launch_trace(p, o.precision, s);
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    {
+        std::lock_guard<std::mutex> lk(g_timing_mu);
+        if (g_timing && hipEventCreate(&ev[0]) == hipSuccess && hipEventCreate(&ev[1]) == hipSuccess)
+            g_timing_events.emplace_back(ev[0], ev[1]);
+    }
+    hipError_t e = launch_trace(p, o.precision, s, ev[1] ? ev[0] : nullptr, ev[1]);
     if (ws)
         hipFreeAsync(ws, s);
     HIP_TRY(e);
     return DMAP_OK;
+}
+
+int dmap_kernel_timing(int enable)
+{
+    std::lock_guard<std::mutex> lk(g_timing_mu);
+    g_timing = enable != 0;
+    return DMAP_OK;
+}
+
+int dmap_kernel_timing_read(double *trace_ms, int64_t *launches)
+{
+    if (!trace_ms || !launches)
+        return fail(DMAP_ERR_INVALID, "dmap_kernel_timing_read: NULL argument");
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> evs;
+    {
+        std::lock_guard<std::mutex> lk(g_timing_mu);
+        evs.swap(g_timing_events);
+    }
+    double sum = 0.0;
+    int rc = DMAP_OK;
+    for (auto &pr : evs) {
+        float ms = 0.f;
+        hipError_t e = hipEventSynchronize(pr.second);
+        if (e == hipSuccess)
+            e = hipEventElapsedTime(&ms, pr.first, pr.second);
+        if (e != hipSuccess && rc == DMAP_OK)
+            rc = fail(DMAP_ERR_HIP, "dmap_kernel_timing_read: %s", hipGetErrorString(e));
+        sum += ms;
+        (void)hipEventDestroy(pr.first);
+        (void)hipEventDestroy(pr.second);
+    }
+    *trace_ms = sum;
+    *launches = (int64_t)evs.size();
+    return rc;
 }
 
 int dmap_intersect_closest(const dmap_object *obj, const dmap_rays *rays, int64_t n, const dmap_hits *hits,

@@ -126,7 +126,10 @@ hipError_t launch_adjoint(const AdjointParams &p, hipStream_t stream);
 // Closest hit / any hit over the object; precision 0 = fp32, 1 = fp64.
 // A closest-hit launch is followed by the finalize kernel, which turns the
 // raw record (t, prim, texel, texel-local root in s/tt) into the hit.
-hipError_t launch_trace(const TraceParams &p, int precision, hipStream_t stream);
+// ev_start / ev_end, when given, are recorded on `stream` right before and
+// after the traversal kernel (for its own duration).
+hipError_t launch_trace(const TraceParams &p, int precision, hipStream_t stream, hipEvent_t ev_start = nullptr,
+                        hipEvent_t ev_end = nullptr);
 // The multi-request closest hit of serving: reset, trace and finalize over
 // the step descriptor at p.step (fp32, Newton leaf, no counters), with a
 // grid that does not depend on the step, so the three launches can be

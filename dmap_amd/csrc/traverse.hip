@@ -1750,7 +1750,8 @@ static void launch_trace_r(const TraceParams &p, unsigned blocks, hipStream_t st
         launch_trace_l<R, STATS, false>(p, blocks, stream);
 }
 
-hipError_t launch_trace(const TraceParams &p, int precision, hipStream_t stream)
+hipError_t launch_trace(const TraceParams &p, int precision, hipStream_t stream, hipEvent_t ev_start,
+                        hipEvent_t ev_end)
 {
     if (p.n_rays <= 0)
         return hipSuccess;
@@ -1761,6 +1762,8 @@ hipError_t launch_trace(const TraceParams &p, int precision, hipStream_t stream)
     hipLaunchKernelGGL(k_queue_reset, dim3(1), dim3(1), 0, stream, p.queue);
     bool stats = p.stats.node_tests || p.stats.leaf_tests || p.stats.sub_squares || p.stats.triangle_tests ||
                  p.stats.newton_runs || p.stats.newton_iters;
+    if (ev_start)
+        (void)hipEventRecord(ev_start, stream);
     if (precision == 1) {
         if (stats)
             launch_trace_r<double, true>(p, blocks, stream);
@@ -1773,6 +1776,8 @@ hipError_t launch_trace(const TraceParams &p, int precision, hipStream_t stream)
             launch_trace_r<float, false>(p, blocks, stream);
     }
     hipError_t e = hipGetLastError();
+    if (ev_end)
+        (void)hipEventRecord(ev_end, stream);
     if (e != hipSuccess || p.any_hit)
         return e;
     unsigned fblocks = (unsigned)((p.n_rays + 255) / 256);

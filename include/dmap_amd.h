@@ -155,6 +155,14 @@ int dmap_intersect_adjoint(const dmap_object *obj, const dmap_rays *rays, int64_
                            const float *dl_dnx, const float *dl_dny, const float *dl_dnz, double *grad_texels,
                            const dmap_options *options, void *stream);
 
+/* Kernel timing for measurement (no counterpart in the reference): while
+ * enabled, every closest/any-hit call records HIP events around its
+ * traversal kernel on its stream; dmap_kernel_timing_read waits for them and
+ * returns the summed kernel time (ms) and the number of launches since the
+ * last read. */
+int dmap_kernel_timing(int enable);
+int dmap_kernel_timing_read(double *trace_ms, int64_t *launches);
+
 /* Serving (continuous batching; no counterpart in the reference, which
  * traces one ray per call). Requests against resident objects are queued,
  * admitted in arrival order into the next free slot while the step's ray
