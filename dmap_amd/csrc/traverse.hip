@@ -1067,9 +1067,11 @@ enum
 
 // Leaves run when at least this share (in 1/64) of the wave's busy lanes
 // have one pending, or when no lane has a block to expand.
-constexpr int LEAF_BATCH = 24;
-// A wave claims new rays when at least this many of its lanes are idle.
-constexpr int REFILL = 16;
+constexpr int LEAF_BATCH = 56;
+// A wave claims new rays when at least this many of its lanes are idle
+// (both thresholds from a sweep on the incoherent 4M-ray workload: fewer,
+// fuller refills and leaf batches keep more lanes active per instruction).
+constexpr int REFILL = 48;
 
 // Closest or any hit, one ray per lane. The walk of each triangle is the
 // reference's depth-first descent from its roots (DisplacedIntersector::
@@ -1713,6 +1715,9 @@ template <class R> static size_t trace_lds_bytes(unsigned block)
     return (size_t)STACK * block * (sizeof(typename StackSlot<R>::T) + sizeof(uint16_t));
 }
 
+// Persistent single-object launches: workgroups per CU.
+constexpr int TRACE_GROUPS_PER_CU = 4;
+
 // Multi-request launches use one-wave workgroups: a workgroup's resources
 // free as soon as its wave is done, so the next step's waves fill the tail
 // of a running step wave by wave.
@@ -1758,7 +1763,7 @@ hipError_t launch_trace(const TraceParams &p, int precision, hipStream_t stream,
     // Persistent waves: enough workgroups to fill every CU a few times over.
     int cus = cu_count();
     int64_t want = (p.n_rays + BLOCK - 1) / BLOCK;
-    unsigned blocks = (unsigned)(want < (int64_t)cus * 4 ? want : (int64_t)cus * 4);
+    unsigned blocks = (unsigned)(want < (int64_t)cus * TRACE_GROUPS_PER_CU ? want : (int64_t)cus * TRACE_GROUPS_PER_CU);
     hipLaunchKernelGGL(k_queue_reset, dim3(1), dim3(1), 0, stream, p.queue);
     bool stats = p.stats.node_tests || p.stats.leaf_tests || p.stats.sub_squares || p.stats.triangle_tests ||
                  p.stats.newton_runs || p.stats.newton_iters;
