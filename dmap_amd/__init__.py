@@ -39,7 +39,7 @@ EXPORTS = ("dmap_last_error", "dmap_version", "dmap_ctx_create", "dmap_ctx_destr
            "dmap_tile_rebuild", "dmap_tile_info", "dmap_tile_read_level", "dmap_tile_read_nodes",
            "dmap_tile_destroy", "dmap_object_create", "dmap_object_triangle_info", "dmap_object_destroy",
            "dmap_intersect_closest", "dmap_intersect_any", "dmap_tile_set_texels", "dmap_object_refresh",
-           "dmap_intersect_adjoint", "dmap_kernel_timing", "dmap_kernel_timing_read", "dmap_server_create", "dmap_server_submit", "dmap_server_poll",
+           "dmap_intersect_adjoint", "dmap_hit_record", "dmap_kernel_timing", "dmap_kernel_timing_read", "dmap_server_create", "dmap_server_submit", "dmap_server_poll",
            "dmap_server_hits", "dmap_server_copy_hits", "dmap_server_release", "dmap_server_idle",
            "dmap_server_destroy")
 
@@ -62,6 +62,11 @@ class _Rays(ctypes.Structure):
 
 class _Hits(ctypes.Structure):
     _fields_ = [(n, _FP) for n in ("t", "s", "tt", "bu", "bv", "nx", "ny", "nz", "prim", "texel_i", "texel_j")]
+
+
+class _UvMap(ctypes.Structure):
+    _fields_ = [("scale_u", ctypes.c_double), ("scale_v", ctypes.c_double), ("offset_u", ctypes.c_double),
+                ("offset_v", ctypes.c_double), ("flip_v", ctypes.c_int)]
 
 
 class _Stats(ctypes.Structure):
@@ -99,6 +104,8 @@ def load_library():
         L.dmap_intersect_any.argtypes = [vp, c.POINTER(_Rays), c.c_int64, vp, c.c_int, c.POINTER(_Options), vp]
         L.dmap_tile_set_texels.argtypes = [vp, vp, c.c_int, vp]
         L.dmap_kernel_timing.argtypes = [c.c_int]
+        L.dmap_hit_record.argtypes = [vp, c.POINTER(_Rays), c.c_int64, c.POINTER(_Hits), c.POINTER(_UvMap), c.c_int,
+                                      vp, vp]
         L.dmap_kernel_timing_read.argtypes = [c.POINTER(c.c_double), c.POINTER(c.c_int64)]
         L.dmap_object_refresh.argtypes = [vp]
         L.dmap_intersect_adjoint.argtypes = [vp, c.POINTER(_Rays), c.c_int64, vp, vp, vp, vp, vp,
@@ -295,6 +302,27 @@ class ObjectIntersector:
             res["stats"] = {k: stats_t[q] for q, k in enumerate(STAT_NAMES)}
         return res
 
+    def hit_record(self, rays, hits, uv_scale=(1.0, 1.0), uv_offset=(0.0, 0.0), flip_v=False, twosided=False,
+                   stream=None):
+        """The renderer's hit record (DisplacedGeometry::surface_point +
+        compute_intersection's frame, object space) for the result of
+        intersect(rays): dict of device tensors p (3, n), ng, dpdu, dpdv,
+        uv (2, n), t, b."""
+        torch = _torch()
+        r, n = self._rays(rays)
+        h = _Hits(hits["t"].data_ptr(), hits["st"][0].data_ptr(), hits["st"][1].data_ptr(),
+                  hits["bary"][0].data_ptr(), hits["bary"][1].data_ptr(), hits["ng"][0].data_ptr(),
+                  hits["ng"][1].data_ptr(), hits["ng"][2].data_ptr(), hits["prim"].data_ptr(),
+                  hits["texel"][0].data_ptr(), hits["texel"][1].data_ptr())
+        m = _UvMap(float(uv_scale[0]), float(uv_scale[1]), float(uv_offset[0]), float(uv_offset[1]),
+                   1 if flip_v else 0)
+        rec = torch.empty((20, n), dtype=torch.float32, device="cuda")
+        _check(_lib.dmap_hit_record(self._h, ctypes.byref(r), n, ctypes.byref(h), ctypes.byref(m),
+                                    1 if twosided else 0, ctypes.c_void_p(rec.data_ptr()), _stream_ptr(stream)),
+               "dmap_hit_record")
+        return {"p": rec[0:3], "ng": rec[3:6], "dpdu": rec[6:9], "dpdv": rec[9:12], "uv": rec[12:14],
+                "t": rec[14:17], "b": rec[17:20]}
+
     def refresh(self):
         """Triangle bounds and BVH from the tile's current pyramid."""
         _check(_lib.dmap_object_refresh(self._h), "dmap_object_refresh")
@@ -412,6 +440,53 @@ class Server:
         if getattr(self, "_h", None) and self._h.value and _lib is not None:
             _lib.dmap_server_destroy(self._h)
             self._h = None
+
+
+MAX_FILTER_PASSES = 64  # DisplacedGeometry::max_filter_passes (displaced_geometry.h:64)
+# A rejected hit's ray resumes this far (relative) past the reported t: the
+# reported t is the double-polished root rounded to fp32, which can sit just
+# below the fp32 walk's own root, and resuming at t itself would find the
+# same root again. Two crossings closer than this count as one.
+FILTER_RESUME_REL = 1e-5
+
+
+def intersect_filtered(obj, rays, accept, options=None, max_passes=MAX_FILTER_PASSES):
+    """Closest hit that `accept` takes: the renderer's filter-resume loop
+    (intersect_callback, displaced_geometry.cpp:49-77) on a batch. `accept`
+    maps a hit dict (as intersect returns, all rays) to a bool tensor (n,);
+    rays whose hit it rejects are traced again with t_min raised to the
+    rejected t (the walk takes t > t_min only; here FILTER_RESUME_REL past
+    it), up to max_passes times, after which they count as misses, as
+    there. The reference counts the passes per triangle, this loop per ray
+    over the object."""
+    torch = _torch()
+    out = {k: v.clone() for k, v in obj.intersect(rays, options=options).items()}
+    cur = rays.clone()
+    pending = torch.isfinite(out["t"])
+    for _ in range(max_passes):
+        rej = pending & ~accept(out).to(torch.bool)
+        if not bool(rej.any()):
+            return out
+        idx = rej.nonzero().squeeze(1)
+        cur[6, idx] = out["t"][idx] * (1.0 + FILTER_RESUME_REL)
+        again = obj.intersect(cur[:, idx].contiguous(), options=options)
+        for k, v in again.items():
+            out[k][..., idx] = v
+        pending = torch.zeros_like(pending)
+        pending[idx] = torch.isfinite(again["t"])
+    rej = pending & ~accept(out).to(torch.bool)
+    if bool(rej.any()):
+        out["t"][rej] = float("inf")
+        out["prim"][rej] = -1
+        for k in ("st", "bary", "ng", "texel"):
+            out[k][:, rej] = 0
+    return out
+
+
+def occluded_filtered(obj, rays, accept, options=None, max_passes=MAX_FILTER_PASSES):
+    """Any hit that `accept` takes (occluded_callback, displaced_geometry.cpp:79-106)."""
+    res = intersect_filtered(obj, rays, accept, options=options, max_passes=max_passes)
+    return _torch().isfinite(res["t"])
 
 
 def kernel_timing(enable):

@@ -934,6 +934,31 @@ int dmap_intersect_adjoint(const dmap_object *obj, const dmap_rays *rays, int64_
     return rc;
 }
 
+int dmap_hit_record(const dmap_object *obj, const dmap_rays *rays, int64_t n, const dmap_hits *hits,
+                    const dmap_uv_map *uv_map, int twosided, float *record, void *stream)
+{
+    if (!obj || !rays || !hits || !record)
+        return fail(DMAP_ERR_INVALID, "dmap_hit_record: NULL argument");
+    if (!hits->prim || !hits->bu || !hits->bv || (twosided && (!rays->dx || !rays->dy || !rays->dz)))
+        return fail(DMAP_ERR_INVALID, "dmap_hit_record: hits prim, bu, bv (and ray directions when twosided) needed");
+    if (n < 0)
+        return fail(DMAP_ERR_INVALID, "dmap_hit_record: negative count");
+    UvMap uv{1.0, 1.0, 0.0, 0.0, 0};
+    if (uv_map) {
+        uv = UvMap{uv_map->scale_u, uv_map->scale_v, uv_map->offset_u, uv_map->offset_v, uv_map->flip_v ? 1 : 0};
+        if (!(uv.scale_u != 0.0 && uv.scale_v != 0.0 && std::isfinite(uv.scale_u) && std::isfinite(uv.scale_v)))
+            return fail(DMAP_ERR_INVALID, "dmap_hit_record: uv scales must be finite and non-zero");
+    }
+    if (n == 0)
+        return DMAP_OK;
+    HIP_TRY(hipSetDevice(obj->ctx->device));
+    RaysDev R{rays->ox, rays->oy, rays->oz, rays->dx, rays->dy, rays->dz, rays->tmin, rays->tmax};
+    HitsDev H{hits->t,  hits->s,  hits->tt,   hits->bu,      hits->bv,     hits->nx,
+              hits->ny, hits->nz, hits->prim, hits->texel_i, hits->texel_j};
+    HIP_TRY(launch_hit_record(obj->view, R, H, n, uv, twosided, record, (hipStream_t)stream));
+    return DMAP_OK;
+}
+
 int dmap_intersect_any(const dmap_object *obj, const dmap_rays *rays, int64_t n, uint8_t *occluded, int prim_mask,
                        const dmap_options *options, void *stream)
 {

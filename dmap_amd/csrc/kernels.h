@@ -123,6 +123,18 @@ struct AdjointParams
 };
 hipError_t launch_adjoint(const AdjointParams &p, hipStream_t stream);
 
+// The renderer's hit record (DisplacedGeometry::surface_point and
+// compute_intersection's frame, displaced_geometry.cpp:126-183) for the hits
+// of a closest-hit launch: 20 fp32 planes of n values each, p xyz, ng xyz,
+// dpdu xyz, dpdv xyz, uv, frame t xyz, frame b xyz (misses: zeros).
+struct UvMap
+{
+    double scale_u, scale_v, offset_u, offset_v;
+    int flip_v;
+};
+hipError_t launch_hit_record(const ObjectDev &obj, const RaysDev &rays, const HitsDev &hits, int64_t n,
+                             const UvMap &uv, int twosided, float *record, hipStream_t stream);
+
 // Closest hit / any hit over the object; precision 0 = fp32, 1 = fp64.
 // A closest-hit launch is followed by the finalize kernel, which turns the
 // raw record (t, prim, texel, texel-local root in s/tt) into the hit.

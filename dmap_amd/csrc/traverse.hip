@@ -1639,6 +1639,147 @@ hipError_t launch_adjoint(const AdjointParams &p, hipStream_t stream)
     return hipGetLastError();
 }
 
+// ------------------------------------------------------------- hit record
+
+// HeightGrid::h and grad (displaced_surface.cpp:12-47) in double over the
+// closed node grid: height and (dh/ds, dh/dt) at tile coordinates (s, t).
+__device__ void field_at(const PyramidView &P, double s, double t, double &h, double &hs, double &ht)
+{
+    const int W = P.n_leaf + 1;
+    if (P.repeat) {
+        s -= floor(s);
+        t -= floor(t);
+    } else {
+        s = fmin(fmax(s, 0.0), 1.0);
+        t = fmin(fmax(t, 0.0), 1.0);
+    }
+    const double fx = s * (W - 1), fy = t * (W - 1);
+    const int i = min((int)fx, W - 2), j = min((int)fy, W - 2);
+    const double a = fx - i, b = fy - j;
+    const float *row0 = P.nodes + (size_t)j * W, *row1 = row0 + W;
+    const double v00 = row0[i], v10 = row0[i + 1], v01 = row1[i], v11 = row1[i + 1];
+    h = P.scale_d * ((1.0 - a) * (1.0 - b) * v00 + a * (1.0 - b) * v10 + (1.0 - a) * b * v01 + a * b * v11) +
+        P.offset_d;
+    hs = P.scale_d * (((1.0 - b) * (v10 - v00) + b * (v11 - v01)) * (W - 1));
+    ht = P.scale_d * (((1.0 - a) * (v01 - v00) + a * (v11 - v10)) * (W - 1));
+}
+
+__global__ void __launch_bounds__(256) k_hit_record(ObjectDev obj, RaysDev rays, HitsDev hits, int64_t n, UvMap uv,
+                                                    int twosided, float *rec)
+{
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n)
+        return;
+    const int prim = hits.prim[r];
+    if (prim < 0) {
+        for (int q = 0; q < 20; ++q)
+            rec[q * n + r] = 0.f;
+        return;
+    }
+    const TriRecord &T = obj.tris[prim];
+    // st = t0 + J^-1 (u, v) (BaseTriangle::param).
+    const double u = hits.bu[r], v = hits.bv[r];
+    const double detJ = T.J[0] * T.J[3] - T.J[1] * T.J[2];
+    const double s = T.t0[0] + (T.J[3] * u - T.J[1] * v) / detJ;
+    const double t = T.t0[1] + (-T.J[2] * u + T.J[0] * v) / detJ;
+    // normal_frame_at (displaced_surface.cpp:99-108).
+    double M[3], N[3], Nu[3], Nv[3];
+    for (int q = 0; q < 3; ++q)
+        M[q] = T.m0[q] + s * T.Mu[q] + t * T.Mv[q];
+    const double len = sqrt(M[0] * M[0] + M[1] * M[1] + M[2] * M[2]);
+    for (int q = 0; q < 3; ++q)
+        N[q] = M[q] / len;
+    const double nu = N[0] * T.Mu[0] + N[1] * T.Mu[1] + N[2] * T.Mu[2];
+    const double nv = N[0] * T.Mv[0] + N[1] * T.Mv[1] + N[2] * T.Mv[2];
+    for (int q = 0; q < 3; ++q) {
+        Nu[q] = (T.Mu[q] - nu * N[q]) / len;
+        Nv[q] = (T.Mv[q] - nv * N[q]) / len;
+    }
+    double h, hs, ht;
+    field_at(obj.pyr, s, t, h, hs, ht);
+    double Ss[3], St[3], p[3];
+    for (int q = 0; q < 3; ++q) {
+        Ss[q] = (T.e1[q] + hs * N[q]) + h * Nu[q];
+        St[q] = (T.e2[q] + ht * N[q]) + h * Nv[q];
+        p[q] = T.q0[q] + s * T.e1[q] + t * T.e2[q] + h * N[q];
+    }
+    // Oriented along the base face's winding: (p1 - p0) x (p2 - p0) = det(J^-1) e1 x e2.
+    double wind[3] = {T.e1[1] * T.e2[2] - T.e1[2] * T.e2[1], T.e1[2] * T.e2[0] - T.e1[0] * T.e2[2],
+                      T.e1[0] * T.e2[1] - T.e1[1] * T.e2[0]};
+    if (detJ < 0.0)
+        for (int q = 0; q < 3; ++q)
+            wind[q] = -wind[q];
+    double ng[3] = {Ss[1] * St[2] - Ss[2] * St[1], Ss[2] * St[0] - Ss[0] * St[2], Ss[0] * St[1] - Ss[1] * St[0]};
+    if (ng[0] * wind[0] + ng[1] * wind[1] + ng[2] * wind[2] < 0.0)
+        for (int q = 0; q < 3; ++q)
+            ng[q] = -ng[q];
+    double nl = sqrt(ng[0] * ng[0] + ng[1] * ng[1] + ng[2] * ng[2]);
+    if (!(nl > 0.0) || !isfinite(nl)) {
+        for (int q = 0; q < 3; ++q)
+            ng[q] = wind[q];
+        nl = sqrt(ng[0] * ng[0] + ng[1] * ng[1] + ng[2] * ng[2]);
+    }
+    for (int q = 0; q < 3; ++q)
+        ng[q] /= nl;
+    double uu = (s - uv.offset_u) / uv.scale_u, vv = (t - uv.offset_v) / uv.scale_v;
+    if (uv.flip_v)
+        vv = 1.0 - vv;
+    const double sv = uv.flip_v ? -uv.scale_v : uv.scale_v;
+    // The geometric frame in fp32, as compute_intersection builds it.
+    float ngf[3], du[3], dv[3], pf[3];
+    for (int q = 0; q < 3; ++q) {
+        ngf[q] = (float)ng[q];
+        du[q] = (float)(Ss[q] * uv.scale_u);
+        dv[q] = (float)(St[q] * sv);
+        pf[q] = (float)p[q];
+    }
+    auto fcross = [](const float *a, const float *b, float *c) {
+        c[0] = a[1] * b[2] - a[2] * b[1];
+        c[1] = a[2] * b[0] - a[0] * b[2];
+        c[2] = a[0] * b[1] - a[1] * b[0];
+    };
+    auto fnormalize = [](float *a) {
+        const float l = sqrtf(a[0] * a[0] + a[1] * a[1] + a[2] * a[2]);
+        a[0] /= l;
+        a[1] /= l;
+        a[2] /= l;
+        return isfinite(a[0]) && isfinite(a[1]) && isfinite(a[2]);
+    };
+    float bf[3], tf[3];
+    fcross(ngf, du, bf);
+    if (!fnormalize(bf)) {
+        const float ax[3] = {fabsf(ngf[0]) < 0.9f ? 1.f : 0.f, fabsf(ngf[0]) < 0.9f ? 0.f : 1.f, 0.f};
+        fcross(ngf, ax, bf);
+        fnormalize(bf);
+    }
+    fcross(bf, ngf, tf);
+    fnormalize(tf);
+    if (twosided) {
+        float dd[3] = {rays.dx[r], rays.dy[r], rays.dz[r]};
+        fnormalize(dd);
+        if (-(dd[0] * ngf[0] + dd[1] * ngf[1] + dd[2] * ngf[2]) < 0.f)
+            for (int q = 0; q < 3; ++q) {
+                ngf[q] = -ngf[q];
+                du[q] = -du[q];
+                tf[q] = -tf[q];
+            }
+    }
+    const float vals[20] = {pf[0],  pf[1],  pf[2], ngf[0], ngf[1], ngf[2], du[0], du[1], du[2], dv[0],
+                            dv[1],  dv[2],  (float)uu, (float)vv, tf[0], tf[1], tf[2], bf[0], bf[1], bf[2]};
+    for (int q = 0; q < 20; ++q)
+        rec[q * n + r] = vals[q];
+}
+
+hipError_t launch_hit_record(const ObjectDev &obj, const RaysDev &rays, const HitsDev &hits, int64_t n,
+                             const UvMap &uv, int twosided, float *record, hipStream_t stream)
+{
+    if (n <= 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(k_hit_record, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, obj, rays, hits, n, uv,
+                       twosided, record);
+    return hipGetLastError();
+}
+
 // ------------------------------------------------------- height range
 
 // classify_square in double (uv_clip.cpp:27-44): 0 outside, 1 inside, 2 straddle.

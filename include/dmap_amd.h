@@ -14,6 +14,8 @@
  *                         and, with prim_mask = k, DisplacedIntersector::intersect
  *                         (displaced_intersector.h:137-138) on triangle k alone
  *   dmap_intersect_any    the occluded callback (displaced_geometry.cpp:79-106)
+ *   dmap_hit_record       DisplacedGeometry::surface_point + compute_intersection's
+ *                         frame (displaced_geometry.cpp:126-183)
  * The adjoint, tile texel updates and the serving loop have no counterpart
  * in the reference (see their declarations).
  *
@@ -142,6 +144,22 @@ int dmap_intersect_closest(const dmap_object *obj, const dmap_rays *rays, int64_
 /* Any hit: occluded[r] = 1 when the ray meets the surface in (tmin, tmax). */
 int dmap_intersect_any(const dmap_object *obj, const dmap_rays *rays, int64_t n, uint8_t *occluded, int prim_mask,
                        const dmap_options *options, void *stream);
+
+/* The renderer's hit record (DisplacedGeometry::surface_point and the
+ * geometric frame of compute_intersection, displaced_geometry.cpp:126-183),
+ * in object space, for the hits of a closest-hit call (needs hits prim, bu,
+ * bv): record holds 20 fp32 planes of n values, in order p xyz, ng xyz
+ * (oriented along the base face's winding; towards -d when twosided), dpdu
+ * xyz, dpdv xyz, uv, frame tangent xyz, frame bitangent xyz. uv_map (NULL:
+ * identity) maps tile coordinates to the object's uv (DisplacedObject::
+ * object_uv, scene_spec.cpp:382-388). The world transform and ray
+ * differentials stay with the renderer. Misses get zeros. */
+typedef struct dmap_uv_map {
+    double scale_u, scale_v, offset_u, offset_v;
+    int flip_v;
+} dmap_uv_map;
+int dmap_hit_record(const dmap_object *obj, const dmap_rays *rays, int64_t n, const dmap_hits *hits,
+                    const dmap_uv_map *uv_map, int twosided, float *record, void *stream);
 
 /* Adjoint of the closest hit (no counterpart in the reference): for the
  * loss gradients dl_dt[r] = dL/dt_r and (dl_dnx, dl_dny, dl_dnz)[r] =

@@ -91,6 +91,7 @@ def _declare(L):
     L.orc_field_eval.argtypes = [_VP, c.c_int64, _D, _D, _D]
     L.orc_classify_square.argtypes = [_D, c.c_double, c.c_double, c.c_double]
     L.orc_surface_point.argtypes = [_VP, c.c_int, c.c_double, c.c_double, _D]
+    L.orc_hit_record.argtypes = [_VP, c.c_int, c.c_double, c.c_double, _D, _D, c.c_int, _D]
     L.orc_scene_adjoint.argtypes = [_VP, c.c_int64, _D, _D, _D, _D, _D, _D, c.c_int, _D, _D, _D]
 
 
@@ -200,6 +201,15 @@ class Scene:
         lib().orc_scene_occluded(self.handle, n, _p(o), _p(d), _p(tmin), _p(tmax), BOUND[bound],
                                  int(slab_max_level), LEAF[leaf], int(threads), _p(occ, _U8))
         return occ.astype(bool)
+
+    def hit_record(self, k, u, v, d, uv_scale=(1.0, 1.0), uv_offset=(0.0, 0.0), flip_v=False, twosided=False):
+        """The renderer's hit record at barycentrics (u, v) of triangle k: dict of p, ng, dpdu, dpdv, uv, t, b."""
+        out = np.empty(20)
+        m = np.array([uv_scale[0], uv_scale[1], uv_offset[0], uv_offset[1], 1.0 if flip_v else 0.0])
+        lib().orc_hit_record(self.handle, int(k), float(u), float(v), _p(_f64(d)), _p(m), 1 if twosided else 0,
+                             _p(out))
+        return {"p": out[0:3], "ng": out[3:6], "dpdu": out[6:9], "dpdv": out[9:12], "uv": out[12:14],
+                "t": out[14:17], "b": out[17:20]}
 
     def adjoint(self, o, d, dl_dt, dl_dn=None, tmin=None, tmax=None, threads=0):
         """Gradient of sum_r dl_dt[r] t_r + dl_dn[r] . ng_r with respect to

@@ -14,6 +14,7 @@
 #include "dmap_oracle.h"
 
 #include <algorithm>
+#include <array>
 #include <atomic>
 #include <cmath>
 #include <cstdint>
@@ -1731,6 +1732,68 @@ int orc_classify_square(const double *t6, double cx, double cy, double s)
     UvTri u;
     u.init(v2(t6[0], t6[1]), v2(t6[2], t6[3]), v2(t6[4], t6[5]));
     return (int)classify_square(u, v2(cx, cy), s);
+}
+
+void orc_hit_record(void *scene, int k, double u, double v, const double *dir3, const double *uv_map, int twosided,
+                    double *out20)
+{
+    // DisplacedGeometry::surface_point (displaced_geometry.cpp:126-152) with the object's uv map
+    // (DisplacedObject::object_uv, scene_spec.cpp:382-388), then compute_intersection's geometric frame
+    // (displaced_geometry.cpp:154-183) in single precision as there, in object space.
+    const Scene *s = (const Scene *)scene;
+    const Tri &tri = s->tris[k];
+    const double su = uv_map[0], sv = uv_map[1], ou = uv_map[2], ov = uv_map[3];
+    const bool flip = uv_map[4] != 0.0;
+    V2 st = tri.param(u, v);
+    NFrame f = normal_frame_at(tri, st.x, st.y);
+    double h = s->grid.h(st.x, st.y);
+    V2 gh = s->grid.grad(st.x, st.y);
+    V3 Ss = (tri.e1 + gh.x * f.N) + h * f.Nu;
+    V3 St = (tri.e2 + gh.y * f.N) + h * f.Nv;
+    V3 p = tri.P(st.x, st.y) + h * f.N;
+    V3 wind = cross(tri.p1 - tri.p0, tri.p2 - tri.p0);
+    V3 ng = cross(Ss, St);
+    if (dot(ng, wind) < 0.0)
+        ng = -ng;
+    ng = normalized(ng);
+    if (!(std::isfinite(ng.x) && std::isfinite(ng.y) && std::isfinite(ng.z)))
+        ng = normalized(wind);
+    V3 dpdu = su * Ss, dpdv = (flip ? -sv : sv) * St;
+    double uu = (st.x - ou) / su, vv = (st.y - ov) / sv;
+    if (flip)
+        vv = 1.0 - vv;
+    // The frame in float.
+    auto f3 = [](V3 a) { return std::array<float, 3>{(float)a.x, (float)a.y, (float)a.z}; };
+    auto fcross = [](const std::array<float, 3> &a, const std::array<float, 3> &b) {
+        return std::array<float, 3>{a[1] * b[2] - a[2] * b[1], a[2] * b[0] - a[0] * b[2], a[0] * b[1] - a[1] * b[0]};
+    };
+    auto fnorm = [](std::array<float, 3> a) {
+        float l = std::sqrt(a[0] * a[0] + a[1] * a[1] + a[2] * a[2]);
+        return std::array<float, 3>{a[0] / l, a[1] / l, a[2] / l};
+    };
+    auto finite = [](const std::array<float, 3> &a) {
+        return std::isfinite(a[0]) && std::isfinite(a[1]) && std::isfinite(a[2]);
+    };
+    std::array<float, 3> ngf = f3(ng), du = f3(dpdu), dv = f3(dpdv);
+    std::array<float, 3> b = fnorm(fcross(ngf, du));
+    if (!finite(b)) {
+        std::array<float, 3> ax = std::abs(ngf[0]) < 0.9f ? std::array<float, 3>{1, 0, 0} : std::array<float, 3>{0, 1, 0};
+        b = fnorm(fcross(ngf, ax));
+    }
+    std::array<float, 3> t = fnorm(fcross(b, ngf));
+    if (twosided) {
+        std::array<float, 3> dd = fnorm(std::array<float, 3>{(float)dir3[0], (float)dir3[1], (float)dir3[2]});
+        float wo_ng = -(dd[0] * ngf[0] + dd[1] * ngf[1] + dd[2] * ngf[2]);
+        if (wo_ng < 0.0f)
+            for (int q = 0; q < 3; ++q) {
+                ngf[q] = -ngf[q];
+                du[q] = -du[q];
+                t[q] = -t[q];
+            }
+    }
+    double vals[20] = {p.x,   p.y,   p.z,   ngf[0], ngf[1], ngf[2], du[0], du[1], du[2], dv[0],
+                       dv[1], dv[2], uu,    vv,     t[0],   t[1],   t[2],  b[0],  b[1],  b[2]};
+    std::memcpy(out20, vals, sizeof vals);
 }
 
 void orc_surface_point(void *scene, int k, double u, double v, double *out12)

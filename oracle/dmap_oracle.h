@@ -110,6 +110,12 @@ int orc_classify_square(const double *t6, double cx, double cy, double s);
  * position (3), oriented unit normal (3), dpdu (3), dpdv (3) with unit uv
  * scale. */
 void orc_surface_point(void *scene, int k, double u, double v, double *out12);
+/* The renderer's hit record at barycentrics (u, v) of triangle k
+ * (DisplacedGeometry::surface_point + compute_intersection's frame, object
+ * space): p, ng, dpdu, dpdv, uv, frame t, frame b (20 doubles). uv_map =
+ * (scale_u, scale_v, offset_u, offset_v, flip_v). */
+void orc_hit_record(void *scene, int k, double u, double v, const double *dir3, const double *uv_map, int twosided,
+                    double *out20);
 
 /* Adjoint of the closest hit (no counterpart in the reference): adds into
  * grad_nodes (nodes x nodes, row-major [j][i]) the gradient of

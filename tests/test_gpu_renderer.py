@@ -1,0 +1,124 @@
+"""Renderer-facing calls: the hit record (dmap_hit_record) against the
+oracle's restatement of DisplacedGeometry::surface_point + compute_intersection
+at the same hits, and the batched filter-resume loop (intersect_filtered)
+against its independent equivalents: accept-all = the plain closest hit,
+reject-triangle-k = the object without triangle k, and a texel-parity filter
+= the same loop run on the oracle."""
+import numpy as np
+import pytest
+
+import oracle
+from dmap_amd import workload as W
+from tests import scenes as S
+
+pytestmark = pytest.mark.gpu
+
+
+def _setup(kind, torch, n_tex=64, res=96):
+    import dmap_amd
+    rng = np.random.default_rng(12)
+    tex = rng.random((n_tex, n_tex)).astype(np.float32)
+    tris = {"flat": W.quad_plane(2, 1.0), "curved": S.curved_quad(), "rotated": S.rotated_chart_quad()}[kind]
+    tile = dmap_amd.DisplacementTile(tex, 0.05)
+    obj = dmap_amd.ObjectIntersector(tile, tris)
+    o, d = W.pinhole_rays((0.5, -0.8, 0.9), (0.5, 0.5, 0.0), 40.0, res, res)
+    rays = torch.from_numpy(W.pack_rays(o, d)).cuda()
+    sc = oracle.Scene(W.close_tile(tex).astype(np.float64), 0.05, 0.0, True, tris)
+    return dmap_amd, tile, obj, rays, sc, tris, o, d
+
+
+@pytest.mark.parametrize("kind", ["flat", "curved", "rotated"])
+@pytest.mark.parametrize("twosided", [False, True])
+def test_hit_record_matches_oracle(torch_gpu, kind, twosided):
+    torch = torch_gpu
+    dmap_amd, tile, obj, rays, sc, tris, o, d = _setup(kind, torch)
+    hits = obj.intersect(rays)
+    uvmap = ((2.0, 0.5), (0.1, -0.2), True)
+    rec = obj.hit_record(rays, hits, uv_scale=uvmap[0], uv_offset=uvmap[1], flip_v=uvmap[2], twosided=twosided)
+    R = {k: v.cpu().numpy() for k, v in rec.items()}
+    prim = hits["prim"].cpu().numpy()
+    bary = hits["bary"].cpu().numpy().astype(np.float64)
+    hit = np.flatnonzero(prim >= 0)
+    assert len(hit) > 1000
+    miss = np.flatnonzero(prim < 0)
+    assert all(np.all(R[k][:, miss] == 0) for k in R)
+    for r in hit[:: max(1, len(hit) // 600)]:
+        ref = sc.hit_record(int(prim[r]), bary[0, r], bary[1, r], d[r].astype(np.float64), *uvmap, twosided=twosided)
+        scale = 1 + np.abs(ref["p"]).max()
+        assert np.allclose(R["p"][:, r], ref["p"], atol=2e-7 * scale)
+        assert np.allclose(R["ng"][:, r], ref["ng"], atol=2e-6)
+        for k in ("dpdu", "dpdv"):
+            assert np.allclose(R[k][:, r], ref[k], rtol=2e-6, atol=2e-6 * np.abs(ref[k]).max())
+        assert np.allclose(R["uv"][:, r], ref["uv"], atol=1e-6)
+        assert np.allclose(R["t"][:, r], ref["t"], atol=2e-5) and np.allclose(R["b"][:, r], ref["b"], atol=2e-5)
+    # The record's p is the hit point o + t d, to fp32 resolution.
+    t = hits["t"].cpu().numpy()
+    p_ray = o[hit].astype(np.float64) + t[hit, None] * d[hit].astype(np.float64)
+    assert np.max(np.abs(p_ray - R["p"][:, hit].T)) < 5e-6
+
+
+def test_filtered_accept_all_and_without_triangle(torch_gpu):
+    torch = torch_gpu
+    dmap_amd, tile, obj, rays, sc, tris, o, d = _setup("flat", torch, res=128)
+    plain = obj.intersect(rays)
+    allf = dmap_amd.intersect_filtered(obj, rays, lambda h: torch.ones_like(h["t"], dtype=torch.bool))
+    for k in plain:
+        assert torch.equal(plain[k], allf[k]), k
+    k = 3
+    keep = [q for q in range(len(tris)) if q != k]
+    obj_wo = dmap_amd.ObjectIntersector(tile, tris[keep])
+    ref = obj_wo.intersect(rays)
+    filt = dmap_amd.intersect_filtered(obj, rays, lambda h: h["prim"] != k)
+    assert bool(((plain["prim"] == k).sum()) > 100)
+    # the object without k numbers its triangles 0..n-2
+    remap = torch.tensor(keep + [-1], dtype=torch.int32, device="cuda")
+    ref_prim = remap[torch.where(ref["prim"] >= 0, ref["prim"], torch.full_like(ref["prim"], len(keep)))]
+    assert torch.equal(filt["prim"], ref_prim)
+    for key in ("t", "st", "bary", "ng", "texel"):
+        assert torch.equal(filt[key], ref[key]), key
+    occ = dmap_amd.occluded_filtered(obj, rays, lambda h: h["prim"] != k)
+    assert torch.equal(occ, torch.isfinite(ref["t"]))
+
+
+def test_filtered_texel_parity_matches_oracle_loop(torch_gpu):
+    torch = torch_gpu
+    dmap_amd, tile, obj, rays, sc, tris, o, d = _setup("flat", torch, n_tex=16, res=64)
+    # Grazing rays cross many texels: reject hits on texels with i + j even.
+    rng = np.random.default_rng(4)
+    n = 4096
+    og = np.column_stack([rng.uniform(0, 1, n), rng.uniform(0, 1, n), np.full(n, 0.2)])
+    tg = np.column_stack([rng.uniform(0, 1, n), rng.uniform(0, 1, n), np.zeros(n)])
+    dg = tg - og
+    dg[:, 2] *= 0.3
+    rg = torch.from_numpy(W.pack_rays(og.astype(np.float32), dg.astype(np.float32))).cuda()
+    accept = lambda h: (h["texel"][0] + h["texel"][1]) % 2 == 1  # noqa: E731
+    filt = dmap_amd.intersect_filtered(obj, rg, accept)
+    # The same loop on the oracle.
+    of, df = og.astype(np.float32).astype(np.float64), dg.astype(np.float32).astype(np.float64)
+    tmin = np.zeros(n)
+    t_ref = np.full(n, np.inf)
+    texel_ref = np.zeros((n, 2), dtype=np.int64)
+    todo = np.arange(n)
+    for _ in range(64):
+        if len(todo) == 0:
+            break
+        r = sc.intersect(of[todo], df[todo], tmin[todo])
+        hit = np.isfinite(r["t"])
+        ok = hit & ((r["texel"][:, 0] + r["texel"][:, 1]) % 2 == 1)
+        t_ref[todo[ok]] = r["t"][ok]
+        texel_ref[todo[ok]] = r["texel"][ok]
+        again = hit & ~ok
+        tmin[todo[again]] = (r["t"][again].astype(np.float32) * np.float32(1 + dmap_amd.FILTER_RESUME_REL))
+        todo = todo[again]
+    tg_ = filt["t"].cpu().numpy()
+    hit_g, hit_r = np.isfinite(tg_), np.isfinite(t_ref)
+    flips = int((hit_g != hit_r).sum())
+    assert flips <= n * 1e-3, flips
+    both = hit_g & hit_r
+    assert both.sum() > 150
+    same_texel = np.all(filt["texel"].cpu().numpy().T[both] == texel_ref[both], axis=1)
+    assert same_texel.mean() > 0.999
+    assert np.max(np.abs(tg_[both][same_texel] - t_ref[both][same_texel]) / t_ref[both][same_texel]) < 1e-4
+    # every accepted GPU hit satisfies the filter
+    tex_g = filt["texel"].cpu().numpy()
+    assert np.all(((tex_g[0] + tex_g[1]) % 2 == 1)[hit_g])
