@@ -62,6 +62,7 @@ def main():
     ap.add_argument("--serve-slots", type=int, default=4)
     ap.add_argument("--serve-step", type=int, default=1 << 18, help="rays per serving step")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint timing section")
+    ap.add_argument("--no-pathtrace", action="store_true", help="skip the path-tracing timing section")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run device-to-device copies of MIB MiB (the PMC byte-counter reference) and exit")
     args = ap.parse_args()
@@ -193,6 +194,8 @@ def main():
 
     if not args.no_adjoint:
         extra["adjoint"] = adjoint(log)
+    if not args.no_pathtrace:
+        extra["pathtrace"] = pathtrace(log)
     if args.serve_seconds > 0:
         extra["serving"] = serve(args, log)
 
@@ -220,6 +223,31 @@ def calibrate_copy(mib, repeats=4):
         dst.copy_(src)
     torch.cuda.synchronize()
     print(json.dumps({"copy_bytes": mib << 20, "repeats": repeats}), flush=True)
+
+
+def pathtrace(log):
+    """Config 3: wavefront path tracing (1024^2 pixels, 4 spp, 4 bounces, NEE
+    with MIS on a 2 x 2 area light, environment 0.2, albedo 0.7) over a
+    4096^2 displaced plane; closest-hit and any-hit batches per bounce."""
+    import torch
+
+    from dmap_amd import pathtrace as PT
+
+    scene, cam_o, cam_d = PT.c3_scene(torch)
+    PT.render(torch, scene, cam_o[:, :4096], cam_d[:, :4096], spp=1, depth=1)
+    torch.cuda.synchronize()
+    st = {}
+    t0 = time.perf_counter()
+    img = PT.render(torch, scene, cam_o, cam_d, spp=4, depth=4, seed=0, stats=st)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    rays = st["closest_rays"] + st["shadow_rays"]
+    log(f"[pathtrace] 1024^2 x 4 spp, 4 bounces: {dt * 1e3:.1f} ms, {st['closest_rays']} closest + "
+        f"{st['shadow_rays']} shadow rays, {rays / dt / 1e6:.1f} Mrays/s, mean radiance {img.mean().item():.4f}")
+    return {"workload": "1024^2 pixels x 4 spp, 4 bounces, 4096^2 fBm+ridged on 64 quads, NEE+MIS rect light 10, "
+                        "env 0.2, albedo 0.7",
+            "ms": round(dt * 1e3, 2), "closest_rays": st["closest_rays"], "shadow_rays": st["shadow_rays"],
+            "mrays_s": round(rays / dt / 1e6, 2), "mean_radiance": round(img.mean().item(), 5)}
 
 
 def adjoint(log, repeats=10):
