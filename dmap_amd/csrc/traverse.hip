@@ -1072,6 +1072,10 @@ constexpr int LEAF_BATCH = 56;
 // (both thresholds from a sweep on the incoherent 4M-ray workload: fewer,
 // fuller refills and leaf batches keep more lanes active per instruction).
 constexpr int REFILL = 48;
+// The same two thresholds for multi-request (serving) launches, whose rays
+// are not sorted.
+constexpr int MULTI_LEAF_BATCH = 56;
+constexpr int MULTI_REFILL = 48;
 
 // Closest or any hit, one ray per lane. The walk of each triangle is the
 // reference's depth-first descent from its roots (DisplacedIntersector::
@@ -1166,7 +1170,7 @@ k_trace(TraceParams p)
         }
         if (!drained) {
             uint64_t idle = __ballot(!have);
-            if (__popcll(idle) >= REFILL || idle == ~0ull) {
+            if (__popcll(idle) >= (MULTI ? MULTI_REFILL : REFILL) || idle == ~0ull) {
                 int cnt = __popcll(idle);
                 int leader = __ffsll((unsigned long long)idle) - 1;
                 unsigned base = 0;
@@ -1334,7 +1338,7 @@ k_trace(TraceParams p)
         uint64_t leaf_lanes = __ballot(state == ST_LEAF);
         uint64_t busy_lanes = __ballot(state != ST_DONE);
         if (leaf_lanes && (leaf_lanes == busy_lanes ||
-                           64 * __popcll(leaf_lanes) >= LEAF_BATCH * __popcll(busy_lanes))) {
+                           64 * __popcll(leaf_lanes) >= (MULTI ? MULTI_LEAF_BATCH : LEAF_BATCH) * __popcll(busy_lanes))) {
             if (state == ST_LEAF) {
                 if (STATS)
                     ++st.leaf;
