@@ -35,7 +35,8 @@ CHANNELS = ("h0", "gu", "gv", "r", "ru", "rv", "h_min", "h_max")
 STAT_NAMES = ("triangle_tests", "node_tests", "leaf_tests", "sub_squares", "newton_runs", "newton_iters")
 
 # Exported symbols of include/dmap_amd.h (checked by the CPU tests).
-EXPORTS = ("dmap_last_error", "dmap_version", "dmap_ctx_create", "dmap_ctx_destroy", "dmap_tile_create",
+EXPORTS = ("dmap_last_error", "dmap_version", "dmap_ctx_create", "dmap_ctx_destroy", "dmap_device_alloc",
+           "dmap_device_free", "dmap_memcpy", "dmap_stream_synchronize", "dmap_tile_create",
            "dmap_tile_rebuild", "dmap_tile_info", "dmap_tile_read_level", "dmap_tile_read_nodes",
            "dmap_tile_destroy", "dmap_object_create", "dmap_object_triangle_info", "dmap_object_destroy",
            "dmap_intersect_closest", "dmap_intersect_any", "dmap_tile_set_texels", "dmap_object_refresh",

@@ -63,5 +63,31 @@ def build(verbose=False):
     return LIB
 
 
+ROOT = HERE.parent
+CPP_TESTS = {"validate_traversal_gpu": ROOT / "tests" / "cpp" / "validate_traversal_gpu.cpp"}
+
+
+def build_cpp_tests(verbose=False):
+    """The C++ programs that exercise the library through include/dmap_amd.hpp
+    (tests/cpp/, linked against the built library with g++), into
+    tests/cpp/_build/."""
+    out_dir = ROOT / "tests" / "cpp" / "_build"
+    out_dir.mkdir(parents=True, exist_ok=True)
+    gxx = shutil.which("g++") or "g++"
+    built = []
+    for name, src in CPP_TESTS.items():
+        exe = out_dir / name
+        cmd = [gxx, "-O2", "-std=c++17", f"-I{ROOT / 'include'}", str(src), f"-L{OUT_DIR}", "-ldmap_amd",
+               "-Wl,-rpath,$ORIGIN/../../../dmap_amd/_lib", "-o", str(exe)]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"compile failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+        built.append(exe)
+    return built
+
+
 if __name__ == "__main__":
     print(build(verbose="-v" in sys.argv))
+    print(build_cpp_tests(verbose="-v" in sys.argv))

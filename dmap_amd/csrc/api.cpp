@@ -404,6 +404,52 @@ int dmap_ctx_create(int device, dmap_ctx **out)
 
 void dmap_ctx_destroy(dmap_ctx *ctx) { delete ctx; }
 
+int dmap_device_alloc(dmap_ctx *ctx, int64_t bytes, void **ptr)
+{
+    if (!ctx || !ptr || bytes < 0)
+        return fail(DMAP_ERR_INVALID, "dmap_device_alloc: bad argument");
+    *ptr = nullptr;
+    if (bytes == 0)
+        return DMAP_OK;
+    HIP_TRY(hipSetDevice(ctx->device));
+    hipError_t e = hipMalloc(ptr, (size_t)bytes);
+    if (e != hipSuccess)
+        return fail(DMAP_ERR_MEMORY, "dmap_device_alloc: %lld bytes: %s", (long long)bytes, hipGetErrorString(e));
+    return DMAP_OK;
+}
+
+int dmap_device_free(dmap_ctx *ctx, void *ptr)
+{
+    if (!ctx)
+        return fail(DMAP_ERR_INVALID, "dmap_device_free: NULL context");
+    if (!ptr)
+        return DMAP_OK;
+    HIP_TRY(hipSetDevice(ctx->device));
+    HIP_TRY(hipFree(ptr));
+    return DMAP_OK;
+}
+
+int dmap_memcpy(void *dst, const void *src, int64_t bytes, int kind, void *stream)
+{
+    if (bytes < 0 || kind < 0 || kind > 2 || (bytes > 0 && (!dst || !src)))
+        return fail(DMAP_ERR_INVALID, "dmap_memcpy: bad argument");
+    if (bytes == 0)
+        return DMAP_OK;
+    const hipMemcpyKind k[3] = {hipMemcpyHostToDevice, hipMemcpyDeviceToHost, hipMemcpyDeviceToDevice};
+    if (stream) {
+        HIP_TRY(hipMemcpyAsync(dst, src, (size_t)bytes, k[kind], (hipStream_t)stream));
+    } else {
+        HIP_TRY(hipMemcpy(dst, src, (size_t)bytes, k[kind]));
+    }
+    return DMAP_OK;
+}
+
+int dmap_stream_synchronize(void *stream)
+{
+    HIP_TRY(hipStreamSynchronize((hipStream_t)stream));
+    return DMAP_OK;
+}
+
 static void tile_free(dmap_tile *t)
 {
     if (!t)

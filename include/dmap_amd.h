@@ -94,6 +94,15 @@ const char *dmap_version(void);
 int dmap_ctx_create(int device, dmap_ctx **out);
 void dmap_ctx_destroy(dmap_ctx *ctx);
 
+/* Device memory for callers without a HIP toolchain of their own (the C++
+ * API in dmap_amd.hpp uses these): allocate and free on the context's
+ * device, copy (kind 0 host to device, 1 device to host, 2 device to
+ * device; on `stream`, synchronous when stream is NULL), wait for a stream. */
+int dmap_device_alloc(dmap_ctx *ctx, int64_t bytes, void **ptr);
+int dmap_device_free(dmap_ctx *ctx, void *ptr);
+int dmap_memcpy(void *dst, const void *src, int64_t bytes, int kind, void *stream);
+int dmap_stream_synchronize(void *stream);
+
 /* A resident displacement tile from n x n texels (n a power of two):
  * h = strength * texel + offset (HeightGrid, displaced_surface.h:32-47),
  * repeat 1 wraps, 0 clamps; build 0 fold, 1 direct (PyramidBuild,

@@ -44,3 +44,16 @@ def test_product_does_not_touch_oracle():
             text = p.read_text()
             assert "import oracle" not in text and "from oracle" not in text, p
             assert "dmap_oracle" not in text, p
+
+
+def test_cpp_header_compiles():
+    """include/dmap_amd.hpp (the C++ API) compiles on its own with g++."""
+    import shutil
+    import subprocess
+    from pathlib import Path
+    root = Path(__file__).resolve().parents[1]
+    gxx = shutil.which("g++")
+    assert gxx
+    r = subprocess.run([gxx, "-std=c++17", "-fsyntax-only", "-x", "c++", f"-I{root / 'include'}", "-"],
+                       input='#include "dmap_amd.hpp"\nint main() { return 0; }\n', capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
