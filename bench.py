@@ -63,6 +63,7 @@ def main():
     ap.add_argument("--serve-step", type=int, default=1 << 19, help="rays per serving step")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint timing section")
     ap.add_argument("--no-pathtrace", action="store_true", help="skip the path-tracing timing section")
+    ap.add_argument("--no-fp64", action="store_true", help="skip the fp64 device run and the fp32 error report")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run device-to-device copies of MIB MiB (the PMC byte-counter reference) and exit")
     args = ap.parse_args()
@@ -135,6 +136,8 @@ def main():
     cpu = None
     roof = None
     extra = {}
+    if args.precision == "fp32" and not args.no_fp64:
+        extra["fp32_vs_fp64"] = fp64_report(obj, rays, hits, wl, tile.n_leaf, log)
     if not args.no_cpu:
         import oracle
         idx = np.arange(0, n, args.cpu_sample)
@@ -211,6 +214,46 @@ def main():
                              "GB_s": round(build_bytes / (build_ms * 1e-3) / 1e9, 1)},
            **extra}
     print(json.dumps(out), flush=True)
+
+
+def fp64_report(obj, rays, hits32, wl, n_leaf, log):
+    """The same rays through the fp64 build of the kernels, outside the timed
+    region: its rate, and the error of the timed fp32 results against it,
+    with the bands of the parity tests (grazing |cos| < 1e-2, domain edge
+    1e-5, texel-edge creases for the normal)."""
+    import torch
+
+    import dmap_amd
+    from tests.parity import compare
+
+    n = rays.shape[1]
+    opt64 = dmap_amd.TraversalOptions(precision="fp64")
+    h64 = obj.alloc_hits(n)
+    obj.intersect(rays, options=opt64, out=h64)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    obj.intersect(rays, options=opt64, out=h64)
+    torch.cuda.synchronize()
+    ms64 = (time.perf_counter() - t0) * 1e3
+
+    def host(h, dtype):
+        F, I = h["f"].cpu().numpy(), h["i"].cpu().numpy()
+        return {"t": F[0].astype(dtype), "st": F[1:3].T.astype(dtype), "ng": F[5:8].T.astype(dtype), "prim": I[0]}
+
+    g32, g64 = host(hits32, np.float32), host(h64, np.float64)
+    d = wl.rays[3:6].T.astype(np.float64)
+    c = compare(g32, g64, d, wl.triangles, n_leaf)
+    both = np.isfinite(g32["t"]) & np.isfinite(g64["t"])
+    t_rel = np.abs(g32["t"][both] - g64["t"][both]) / np.abs(g64["t"][both])
+    out = {"fp64_ms": round(ms64, 3), "fp64_mrays_s": round(n / (ms64 * 1e-3) / 1e6, 1),
+           "hitmiss_flips": c["hitmiss_flips"], "flips_in_band": c["flips_in_band"],
+           "unexplained": c["unexplained"], "t_rel_max": c["worst_t_rel"],
+           "t_rel_p99": float(np.quantile(t_rel, 0.99)) if both.any() else 0.0,
+           "st_abs_max": c["worst_st"], "ng_abs_max": c["worst_ng"], "normal_on_crease": c["normal_on_crease"]}
+    log(f"[fp64] {ms64:.2f} ms ({out['fp64_mrays_s']} Mrays/s); fp32 vs fp64: {out['hitmiss_flips']} hit/miss flips "
+        f"({out['flips_in_band']} in the bands), {out['unexplained']} outside the bar, t rel max {out['t_rel_max']:.2e}"
+        f" p99 {out['t_rel_p99']:.2e}, st {out['st_abs_max']:.2e}, ng {out['ng_abs_max']:.2e}")
+    return out
 
 
 def calibrate_copy(mib, repeats=4):
