@@ -271,26 +271,42 @@ def calibrate_copy(mib, repeats=4):
 def pathtrace(log):
     """Config 3: wavefront path tracing (1024^2 pixels, 4 spp, 4 bounces, NEE
     with MIS on a 2 x 2 area light, environment 0.2, albedo 0.7) over a
-    4096^2 displaced plane; closest-hit and any-hit batches per bounce."""
+    4096^2 displaced plane; closest-hit and any-hit batches per bounce. The
+    device wavefront (dmap_pathtrace) is timed, and the torch loop over the
+    same intersector beside it."""
     import torch
 
+    import dmap_amd
     from dmap_amd import pathtrace as PT
 
     scene, cam_o, cam_d = PT.c3_scene(torch)
-    PT.render(torch, scene, cam_o[:, :4096], cam_d[:, :4096], spp=1, depth=1)
+    PT.render_device(scene, cam_o[:, :4096], cam_d[:, :4096], spp=1, depth=1)
     torch.cuda.synchronize()
     st = {}
+    dmap_amd.kernel_timing_read()
+    dmap_amd.kernel_timing(True)
     t0 = time.perf_counter()
-    img = PT.render(torch, scene, cam_o, cam_d, spp=4, depth=4, seed=0, stats=st)
+    img = PT.render_device(scene, cam_o, cam_d, spp=4, depth=4, seed=0, stats=st)
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
+    dmap_amd.kernel_timing(False)
+    trace_ms, launches = dmap_amd.kernel_timing_read()
     rays = st["closest_rays"] + st["shadow_rays"]
-    log(f"[pathtrace] 1024^2 x 4 spp, 4 bounces: {dt * 1e3:.1f} ms, {st['closest_rays']} closest + "
-        f"{st['shadow_rays']} shadow rays, {rays / dt / 1e6:.1f} Mrays/s, mean radiance {img.mean().item():.4f}")
+    t0 = time.perf_counter()
+    img_t = PT.render(torch, scene, cam_o, cam_d, spp=4, depth=4, seed=0)
+    torch.cuda.synchronize()
+    dt_t = time.perf_counter() - t0
+    same = (torch.abs(img - img_t) <= 1e-9 * (1 + torch.abs(img_t))).double().mean().item()
+    log(f"[pathtrace] 1024^2 x 4 spp, 4 bounces: device wavefront {dt * 1e3:.1f} ms ({trace_ms:.1f} ms in "
+        f"{launches} traversal launches), {st['closest_rays']} closest + {st['shadow_rays']} shadow rays, "
+        f"{rays / dt / 1e6:.1f} Mrays/s, mean radiance {img.mean().item():.4f}; torch loop {dt_t * 1e3:.1f} ms, "
+        f"same pixels {same:.4f}")
     return {"workload": "1024^2 pixels x 4 spp, 4 bounces, 4096^2 fBm+ridged on 64 quads, NEE+MIS rect light 10, "
                         "env 0.2, albedo 0.7",
-            "ms": round(dt * 1e3, 2), "closest_rays": st["closest_rays"], "shadow_rays": st["shadow_rays"],
-            "mrays_s": round(rays / dt / 1e6, 2), "mean_radiance": round(img.mean().item(), 5)}
+            "ms": round(dt * 1e3, 2), "traversal_kernel_ms": round(trace_ms, 2), "traversal_launches": launches,
+            "closest_rays": st["closest_rays"], "shadow_rays": st["shadow_rays"],
+            "mrays_s": round(rays / dt / 1e6, 2), "mean_radiance": round(img.mean().item(), 5),
+            "torch_loop_ms": round(dt_t * 1e3, 2), "torch_loop_same_pixels": round(same, 5)}
 
 
 def adjoint(log, repeats=10):

@@ -42,7 +42,7 @@ EXPORTS = ("dmap_last_error", "dmap_version", "dmap_ctx_create", "dmap_ctx_destr
            "dmap_intersect_closest", "dmap_intersect_any", "dmap_tile_set_texels", "dmap_object_refresh",
            "dmap_intersect_adjoint", "dmap_hit_record", "dmap_kernel_timing", "dmap_kernel_timing_read", "dmap_server_create", "dmap_server_submit", "dmap_server_poll",
            "dmap_server_hits", "dmap_server_copy_hits", "dmap_server_release", "dmap_server_idle",
-           "dmap_server_destroy")
+           "dmap_server_destroy", "dmap_pathtrace")
 
 
 class DmapError(RuntimeError):
@@ -72,6 +72,12 @@ class _UvMap(ctypes.Structure):
 
 class _Stats(ctypes.Structure):
     _fields_ = [(n, _FP) for n in STAT_NAMES]
+
+
+class _PtScene(ctypes.Structure):
+    _fields_ = [("light_corner", ctypes.c_double * 3), ("light_u", ctypes.c_double * 3),
+                ("light_v", ctypes.c_double * 3), ("light_n", ctypes.c_double * 3), ("radiance", ctypes.c_double),
+                ("env", ctypes.c_double), ("albedo", ctypes.c_double)]
 
 
 _lib = None
@@ -111,6 +117,9 @@ def load_library():
         L.dmap_object_refresh.argtypes = [vp]
         L.dmap_intersect_adjoint.argtypes = [vp, c.POINTER(_Rays), c.c_int64, vp, vp, vp, vp, vp,
                                              c.POINTER(_Options), vp]
+        L.dmap_pathtrace.argtypes = [vp, vp, vp, c.c_int64, c.c_int, c.c_int, c.c_uint32, c.c_int,
+                                     c.POINTER(_PtScene), c.POINTER(_Options), vp, c.POINTER(c.c_int64),
+                                     c.POINTER(c.c_int64), vp]
         _lib = L
     return _lib
 
@@ -302,6 +311,39 @@ class ObjectIntersector:
         if stats:
             res["stats"] = {k: stats_t[q] for q, k in enumerate(STAT_NAMES)}
         return res
+
+    def pathtrace(self, cam_o, cam_d, spp, depth, light, radiance, env, albedo, seed=0, strategy="mis",
+                  options=None, stream=None):
+        """Wavefront path tracing of this object on the device
+        (dmap_pathtrace): cam_o, cam_d (3, n_pix) float64 device tensors
+        (copied to row-major if they are not);
+        light = (corner, edge_u, edge_v, unit normal of the emitting side).
+        Returns (image (n_pix,) float64 device tensor of mean radiance,
+        closest-hit rays traced, any-hit rays traced)."""
+        torch = _torch()
+        for name, x in (("cam_o", cam_o), ("cam_d", cam_d)):
+            if not (isinstance(x, torch.Tensor) and x.is_cuda and x.dtype == torch.float64 and x.dim() == 2
+                    and x.shape[0] == 3):
+                raise DmapError(f"{name} must be a (3, n_pix) float64 device tensor")
+        cam_o, cam_d = cam_o.contiguous(), cam_d.contiguous()
+        if cam_o.shape != cam_d.shape:
+            raise DmapError("cam_o and cam_d differ in shape")
+        n_pix = cam_o.shape[1]
+        strategies = {"mis": 0, "bsdf": 1}
+        if strategy not in strategies:
+            raise DmapError(f"strategy {strategy!r}: 'mis' or 'bsdf'")
+        c, u, v, n = (np.asarray(x, dtype=np.float64).reshape(3) for x in light)
+        sc = _PtScene((ctypes.c_double * 3)(*c), (ctypes.c_double * 3)(*u), (ctypes.c_double * 3)(*v),
+                      (ctypes.c_double * 3)(*n), float(radiance), float(env), float(albedo))
+        image = torch.empty(n_pix, dtype=torch.float64, device=cam_o.device)
+        n_closest, n_shadow = ctypes.c_int64(), ctypes.c_int64()
+        opt = (options or TraversalOptions())._c()
+        _check(_lib.dmap_pathtrace(self._h, ctypes.c_void_p(cam_o.data_ptr()), ctypes.c_void_p(cam_d.data_ptr()),
+                                   n_pix, int(spp), int(depth), int(seed) & 0xFFFFFFFF, strategies[strategy],
+                                   ctypes.byref(sc), ctypes.byref(opt), ctypes.c_void_p(image.data_ptr()),
+                                   ctypes.byref(n_closest), ctypes.byref(n_shadow), _stream_ptr(stream)),
+               "dmap_pathtrace")
+        return image, n_closest.value, n_shadow.value
 
     def hit_record(self, rays, hits, uv_scale=(1.0, 1.0), uv_offset=(0.0, 0.0), flip_v=False, twosided=False,
                    stream=None):

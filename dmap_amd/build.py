@@ -23,7 +23,7 @@ COMMON = ["-O3", "-fPIC", "-std=c++17", f"--offload-arch={ARCH}", "-Xclang", "-t
           "-packed-fp32-ops", "-Wno-unused-command-line-argument", f"-I{CSRC}"]
 # The pyramid build keeps the reference's double arithmetic bit for bit.
 PER_FILE = {"pyramid.hip": ["-ffp-contract=off"], "traverse.hip": [], "reorder.hip": [], "api.cpp": ["-x", "hip"],
-            "serve.cpp": ["-x", "hip"]}
+            "serve.cpp": ["-x", "hip"], "pathtrace.cpp": ["-x", "hip", "-ffp-contract=off"]}
 
 
 def hipcc():

@@ -187,6 +187,23 @@ def render(torch, scene, cam_o, cam_d, spp=4, depth=4, seed=0, strategy="mis", s
     return image / spp
 
 
+def render_device(scene, cam_o, cam_d, spp=4, depth=4, seed=0, strategy="mis", stats=None):
+    """render() in one library call (dmap_pathtrace): the same estimator,
+    random numbers and arithmetic, with the shading, next-event and
+    continuation steps as device kernels between the batched closest-hit and
+    any-hit calls, all samples of a pixel in one wavefront. scene.its must
+    be a GpuIntersector."""
+    its = scene.its
+    if not isinstance(its, GpuIntersector):
+        raise TypeError("render_device traces through the library: the scene needs a GpuIntersector")
+    image, n_closest, n_shadow = its.obj.pathtrace(cam_o, cam_d, spp, depth, scene.light, scene.radiance, scene.env,
+                                                   scene.albedo, seed=seed, strategy=strategy, options=its.options)
+    if stats is not None:
+        stats["closest_rays"] = n_closest
+        stats["shadow_rays"] = n_shadow
+    return image
+
+
 # ------------------------------------------------------------ intersectors
 
 class GpuIntersector:

@@ -214,6 +214,32 @@ int dmap_server_release(dmap_server *server, uint64_t tag);
 int dmap_server_idle(dmap_server *server, int *idle);
 void dmap_server_destroy(dmap_server *server);
 
+/* Wavefront path tracing over one object (the path-tracing workload's
+ * estimator; no entry point of the reference: there the renderer, ks's
+ * SmallPT through path_trace.cpp:107-129 and direct_lighting.cpp:155-232,
+ * calls the closest-hit and occluded callbacks one ray at a time, and this
+ * call makes the same queries in batches through dmap_intersect_closest and
+ * dmap_intersect_any). Lambertian surface of albedo `albedo`, a one-sided
+ * rectangular light corner + a u + b v (a, b in [0, 1]) of radiance
+ * `radiance` emitting along the unit normal light_n, and a constant
+ * environment `env`. strategy 0: next-event estimation with the power
+ * heuristic against the cosine-sampled BSDF; 1: BSDF sampling only.
+ * Paths of at most `depth` bounces; path id = pixel * spp + sample keys the
+ * random numbers with `seed`. cam_o, cam_d: device, 3 rows of n_pix doubles
+ * (directions need not be unit). image: device, n_pix doubles, set to the
+ * mean radiance per pixel. The counts of closest-hit and any-hit rays
+ * traced go to *closest_rays and *shadow_rays when not NULL. Returns when
+ * the image is complete on `stream`'s timeline (it synchronizes with the
+ * stream once per bounce to size the next batch). */
+typedef struct dmap_pt_scene
+{
+    double light_corner[3], light_u[3], light_v[3], light_n[3];
+    double radiance, env, albedo;
+} dmap_pt_scene;
+int dmap_pathtrace(const dmap_object *obj, const double *cam_o, const double *cam_d, int64_t n_pix, int spp,
+                   int depth, uint32_t seed, int strategy, const dmap_pt_scene *scene, const dmap_options *options,
+                   double *image, int64_t *closest_rays, int64_t *shadow_rays, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

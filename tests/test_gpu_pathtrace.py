@@ -1,9 +1,10 @@
-"""The wavefront path tracer (config 3's estimator) on the batched GPU
-intersector: the same paths on the CPU oracle give the same image (shared
-counter-based random numbers; paths part only where a hit differs), NEE with
-MIS and BSDF-only sampling agree in expectation, and a white furnace (albedo
-1, environment 1, no light) returns at most 1 everywhere and about 1 on
-average."""
+"""The wavefront path tracer (config 3's estimator), in two forms: the torch
+loop over the batched GPU intersector and the device wavefront
+(dmap_pathtrace). The same paths on the CPU oracle give the same image
+(shared counter-based random numbers; paths part only where a hit differs),
+the two GPU forms give the same image, NEE with MIS and BSDF-only sampling
+agree in expectation, and a white furnace (albedo 1, environment 1, no
+light) returns at most 1 everywhere and about 1 on average."""
 import numpy as np
 import pytest
 
@@ -59,3 +60,55 @@ def test_pathtrace_white_furnace(torch_gpu):
     img = img.cpu().numpy()
     assert img.max() <= 1.0 + 1e-12
     assert img.mean() > 0.99, img.mean()
+
+
+def test_pathtrace_device_matches_oracle_paths(torch_gpu):
+    torch = torch_gpu
+    tile, obj, sc, cam_o, cam_d = _small(torch)
+    gpu = PT.Scene(PT.GpuIntersector(obj))
+    cpu = PT.Scene(CpuIntersector(sc))
+    st = {}
+    img_d = PT.render_device(gpu, cam_o.cuda(), cam_d.cuda(), spp=2, depth=4, seed=7, stats=st).cpu().numpy()
+    img_c = PT.render(torch, cpu, cam_o, cam_d, spp=2, depth=4, seed=7).numpy()
+    assert st["closest_rays"] > 3 * cam_o.shape[1] and st["shadow_rays"] > cam_o.shape[1]
+    close = np.abs(img_d - img_c) <= 1e-4 * (1 + np.abs(img_c))
+    assert close.mean() > 0.97, close.mean()
+    assert abs(img_d.mean() - img_c.mean()) <= 1e-2 * img_c.mean()
+
+
+def test_pathtrace_device_matches_torch_loop(torch_gpu):
+    """Same intersector, same random numbers: the device wavefront and the
+    torch loop trace the same rays, so the images agree to rounding (the
+    order of the per-pixel sums differs) and so do the ray counts."""
+    torch = torch_gpu
+    tile, obj, sc, cam_o, cam_d = _small(torch, res=64)
+    scene = PT.Scene(PT.GpuIntersector(obj))
+    o, d = cam_o.cuda(), cam_d.cuda()
+    for strategy in ("mis", "bsdf"):
+        st_d, st_t = {}, {}
+        img_d = PT.render_device(scene, o, d, spp=3, depth=4, seed=11, strategy=strategy, stats=st_d).cpu().numpy()
+        img_t = PT.render(torch, scene, o, d, spp=3, depth=4, seed=11, strategy=strategy, stats=st_t).cpu().numpy()
+        close = np.abs(img_d - img_t) <= 1e-9 * (1 + np.abs(img_t))
+        assert close.mean() > 0.995, (strategy, close.mean())
+        assert abs(st_d["closest_rays"] - st_t["closest_rays"]) <= 1e-3 * st_t["closest_rays"], (st_d, st_t)
+        assert abs(st_d["shadow_rays"] - st_t["shadow_rays"]) <= 1e-3 * max(st_t["shadow_rays"], 1), (st_d, st_t)
+
+
+def test_pathtrace_device_mis_bsdf_and_furnace(torch_gpu):
+    torch = torch_gpu
+    tile, obj, sc, cam_o, cam_d = _small(torch, res=64)
+    o, d = cam_o.cuda(), cam_d.cuda()
+    scene = PT.Scene(PT.GpuIntersector(obj))
+    mis = PT.render_device(scene, o, d, spp=64, depth=3, seed=1, strategy="mis").mean().item()
+    bsdf = PT.render_device(scene, o, d, spp=64, depth=3, seed=2, strategy="bsdf").mean().item()
+    assert abs(mis - bsdf) <= 0.02 * mis, (mis, bsdf)
+    white = PT.Scene(PT.GpuIntersector(obj), albedo=1.0, env=1.0, radiance=0.0, light_corner=(1e3, 1e3, 1e3))
+    img = PT.render_device(white, o[:, :1024].contiguous(), d[:, :1024].contiguous(), spp=4, depth=24, seed=3, strategy="bsdf")
+    img = img.cpu().numpy()
+    assert img.max() <= 1.0 + 1e-12
+    assert img.mean() > 0.99, img.mean()
+    # an empty image and a zero-bounce render
+    empty = PT.render_device(scene, o[:, :0].contiguous(), d[:, :0].contiguous(), spp=2, depth=2)
+    assert empty.numel() == 0
+    direct = PT.render_device(scene, o, d, spp=1, depth=0).cpu().numpy()
+    assert np.all((direct == 0.0) | (direct == scene.env) | (direct == scene.radiance))
