@@ -331,53 +331,73 @@ void triangle_box(const HostTri &tri, double h_min, double h_max, float out[6])
 
 bool is_pow2(int n) { return n >= 1 && (n & (n - 1)) == 0; }
 
-// Top-level BVH over the triangles' boxes: median split of the box centres
-// along the longest axis of their spread, one triangle per leaf. It stands
-// in for the reference's embree scene over the same boxes
-// (ObjectIntersector::build_bvh, displaced_intersector.cpp:540-554).
-int build_bvh_node(std::vector<BvhNode> &nodes, const std::vector<float> &boxes, std::vector<int> &idx, int lo, int hi,
-                   int parent)
+// Top-level BVH over the triangles' boxes, a complete binary tree in heap
+// order: the triangles of a subtree are split at their median box centre
+// along the longest axis of the centres' spread, the lower (m + 1) / 2 going
+// to child 2i, so a subtree of capacity C (a power of two) never receives
+// more than C triangles. It stands in for the reference's embree scene over
+// the same boxes (ObjectIntersector::build_bvh, displaced_intersector.cpp:540-554).
+// Writes the subtree's box (empty: lo > hi) to blo, bhi.
+void build_bvh_node(std::vector<BvhNode> &nodes, std::vector<int> &leaf, const std::vector<float> &boxes,
+                    std::vector<int> &idx, uint32_t node, int lo, int hi, uint32_t P, float blo[3], float bhi[3])
 {
-    int me = (int)nodes.size();
-    nodes.push_back(BvhNode{});
-    BvhNode nd;
-    nd.parent = parent;
-    float blo[3] = {INFINITY, INFINITY, INFINITY}, bhi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (int a = 0; a < 3; ++a) {
+        blo[a] = INFINITY;
+        bhi[a] = -INFINITY;
+    }
+    if (node >= P) {
+        leaf[node - P] = hi > lo ? idx[lo] : -1;
+        if (hi > lo)
+            for (int a = 0; a < 3; ++a) {
+                blo[a] = boxes[6 * idx[lo] + a];
+                bhi[a] = boxes[6 * idx[lo] + 3 + a];
+            }
+        return;
+    }
     float clo[3] = {INFINITY, INFINITY, INFINITY}, chi[3] = {-INFINITY, -INFINITY, -INFINITY};
     for (int q = lo; q < hi; ++q) {
         const float *b = &boxes[6 * idx[q]];
         for (int a = 0; a < 3; ++a) {
-            blo[a] = std::min(blo[a], b[a]);
-            bhi[a] = std::max(bhi[a], b[3 + a]);
             float c = 0.5f * (b[a] + b[3 + a]);
             clo[a] = std::min(clo[a], c);
             chi[a] = std::max(chi[a], c);
         }
     }
-    for (int a = 0; a < 3; ++a) {
-        nd.lo[a] = blo[a];
-        nd.hi[a] = bhi[a];
-    }
-    if (hi - lo == 1) {
-        nd.left = ~idx[lo];
-        nd.right = -1;
-        nd.axis = 0;
-        nodes[me] = nd;
-        return me;
-    }
     int axis = 0;
-    for (int a = 1; a < 3; ++a)
-        if (chi[a] - clo[a] > chi[axis] - clo[axis])
-            axis = a;
-    int mid = (lo + hi) / 2;
-    std::nth_element(idx.begin() + lo, idx.begin() + mid, idx.begin() + hi, [&](int x, int y) {
-        return boxes[6 * x + axis] + boxes[6 * x + 3 + axis] < boxes[6 * y + axis] + boxes[6 * y + 3 + axis];
-    });
+    if (hi - lo >= 2)
+        for (int a = 1; a < 3; ++a)
+            if (chi[a] - clo[a] > chi[axis] - clo[axis])
+                axis = a;
+    int mid = lo + (hi - lo + 1) / 2;
+    if (hi - lo >= 2)
+        std::nth_element(idx.begin() + lo, idx.begin() + mid, idx.begin() + hi, [&](int x, int y) {
+            return boxes[6 * x + axis] + boxes[6 * x + 3 + axis] < boxes[6 * y + axis] + boxes[6 * y + 3 + axis];
+        });
+    BvhNode nd{};
     nd.axis = axis;
-    nd.left = build_bvh_node(nodes, boxes, idx, lo, mid, me);
-    nd.right = build_bvh_node(nodes, boxes, idx, mid, hi, me);
-    nodes[me] = nd;
-    return me;
+    nd.empty = (mid > lo ? 0 : 1) | (hi > mid ? 0 : 2);
+    build_bvh_node(nodes, leaf, boxes, idx, 2 * node, lo, mid, P, nd.lo[0], nd.hi[0]);
+    build_bvh_node(nodes, leaf, boxes, idx, 2 * node + 1, mid, hi, P, nd.lo[1], nd.hi[1]);
+    for (int c = 0; c < 2; ++c)
+        for (int a = 0; a < 3; ++a) {
+            if (nd.empty & (1 << c)) {
+                nd.lo[c][a] = 0.f;
+                nd.hi[c][a] = 0.f;
+                continue;
+            }
+            blo[a] = std::min(blo[a], nd.lo[c][a]);
+            bhi[a] = std::max(bhi[a], nd.hi[c][a]);
+        }
+    nodes[node] = nd;
+}
+
+// Leaves of the heap-ordered BVH over n triangles: the power of two >= n.
+int bvh_leaves_for(int n_tri)
+{
+    int P = 1;
+    while (P < n_tri)
+        P *= 2;
+    return P;
 }
 
 } // namespace
@@ -683,15 +703,19 @@ static int object_bounds(dmap_object *o, const std::vector<HostTri> &tris)
     std::vector<int> idx(n_tri);
     for (int k = 0; k < n_tri; ++k)
         idx[k] = k;
-    o->bvh_host.clear();
-    o->bvh_host.reserve(2 * (size_t)n_tri);
-    build_bvh_node(o->bvh_host, o->boxes_host, idx, 0, n_tri, -1);
+    const int P = o->bvh_leaves;
+    o->bvh_host.assign((size_t)P, BvhNode{});
+    o->bvh_leaf_host.assign((size_t)P, -1);
+    BvhNode &root = o->bvh_host[0];
+    build_bvh_node(o->bvh_host, o->bvh_leaf_host, o->boxes_host, idx, 1, 0, n_tri, (uint32_t)P, root.lo[0],
+                   root.hi[0]);
+    root.empty = 2;
     o->const_normal = 1;
     for (int k = 0; k < n_tri; ++k)
         o->const_normal &= o->host[k].const_normal;
     for (int a = 0; a < 3; ++a) {
-        o->bound_lo[a] = o->bvh_host[0].lo[a];
-        o->bound_hi[a] = o->bvh_host[0].hi[a];
+        o->bound_lo[a] = root.lo[0][a];
+        o->bound_hi[a] = root.hi[0][a];
     }
     o->view.pyr = view_of(tile);
     o->view.tris = o->tris;
@@ -699,9 +723,13 @@ static int object_bounds(dmap_object *o, const std::vector<HostTri> &tris)
     o->view.n_tri = n_tri;
     o->view.const_normal = o->const_normal;
     o->view.bvh = o->bvh;
+    o->view.bvh_leaf = (const int *)(o->bvh + P);
+    o->view.bvh_leaves = P;
     e = hipMemcpy(o->view_dev, &o->view, sizeof(ObjectDev), hipMemcpyHostToDevice);
     if (e == hipSuccess)
-        e = hipMemcpy(o->bvh, o->bvh_host.data(), sizeof(BvhNode) * o->bvh_host.size(), hipMemcpyHostToDevice);
+        e = hipMemcpy(o->bvh, o->bvh_host.data(), sizeof(BvhNode) * P, hipMemcpyHostToDevice);
+    if (e == hipSuccess)
+        e = hipMemcpy(o->bvh + P, o->bvh_leaf_host.data(), sizeof(int) * P, hipMemcpyHostToDevice);
     if (e == hipSuccess)
         e = hipMemcpy(o->boxes, o->boxes_host.data(), sizeof(float) * 6 * n_tri, hipMemcpyHostToDevice);
     if (e == hipSuccess)
@@ -716,6 +744,8 @@ int dmap_object_create(dmap_ctx *ctx, dmap_tile *tile, int n_tri, const double *
 {
     if (!ctx || !tile || !positions || !normals || !uvs || !out || n_tri <= 0)
         return fail(DMAP_ERR_INVALID, "dmap_object_create: bad argument");
+    if (n_tri > (1 << 30))
+        return fail(DMAP_ERR_INVALID, "dmap_object_create: at most 2^30 triangles per object");
     if (tile->ctx->device != ctx->device)
         return fail(DMAP_ERR_INVALID, "dmap_object_create: tile lives on another device");
     HIP_TRY(hipSetDevice(ctx->device));
@@ -744,7 +774,8 @@ int dmap_object_create(dmap_ctx *ctx, dmap_tile *tile, int n_tri, const double *
     o->positions.assign(positions, positions + 9 * (size_t)n_tri);
     o->normals.assign(normals, normals + 9 * (size_t)n_tri);
     o->uvs.assign(uvs, uvs + 6 * (size_t)n_tri);
-    e = hipMalloc(&o->bvh, sizeof(BvhNode) * (2 * (size_t)n_tri - 1));
+    o->bvh_leaves = bvh_leaves_for(n_tri);
+    e = hipMalloc(&o->bvh, (sizeof(BvhNode) + sizeof(int)) * (size_t)o->bvh_leaves);
     if (e == hipSuccess)
         e = hipMalloc(&o->view_dev, sizeof(ObjectDev));
     if (e != hipSuccess) {

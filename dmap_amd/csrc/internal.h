@@ -42,9 +42,11 @@ struct dmap_object
     std::vector<TriRecord> host;
     std::vector<float> boxes_host, h_range_host;
     std::vector<BvhNode> bvh_host;
+    std::vector<int> bvh_leaf_host;
+    int bvh_leaves = 1;
     TriRecord *tris = nullptr;
     float *boxes = nullptr;
-    BvhNode *bvh = nullptr;
+    BvhNode *bvh = nullptr; // bvh_leaves nodes, then bvh_leaves leaf entries (int)
     ObjectDev view{};              // what a single-object launch passes
     ObjectDev *view_dev = nullptr; // the same, in device memory, for multi-request launches
     std::vector<double> positions, normals, uvs; // the input triangles, for dmap_object_refresh

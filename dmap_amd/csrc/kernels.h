@@ -44,14 +44,21 @@ struct StatsDev
     int32_t *triangle_tests, *node_tests, *leaf_tests, *sub_squares, *newton_runs, *newton_iters;
 };
 
-// A node of the object's top-level BVH over the triangles' boxes:
-// internal nodes have two children, a leaf holds one triangle
-// (left = ~triangle, right = -1). Node 0 is the root.
+// The object's top-level BVH over the triangles' boxes: a complete binary
+// tree in heap order with P = 2^k leaves. Node 1 is the root, node i has
+// children 2i and 2i + 1, and each of the leaves i in [P, 2P) holds one
+// triangle or none (bvh_leaf[i - P] = triangle or -1). An internal node
+// holds its two children's boxes, so one 64-byte read serves both tests;
+// node 0 holds the root's own box in lo[0], hi[0]. The traversal needs no
+// parent links or stack: the path to a node is its index.
 struct BvhNode
 {
-    float lo[3], hi[3];
-    int parent, left, right, axis;
+    float lo[2][3], hi[2][3]; // the boxes of children 2i (lower centres along axis) and 2i + 1
+    int axis;                 // the split axis
+    int empty;                // bit c: child 2i + c holds no triangle
+    int pad[2];
 };
+static_assert(sizeof(BvhNode) == 64, "one BVH node is 64 bytes");
 
 // What the traversal reads of one object: its tile and its triangles.
 struct ObjectDev
@@ -59,7 +66,9 @@ struct ObjectDev
     PyramidView pyr;
     const TriRecord *tris;
     const float *boxes; // n_tri x 6: lo xyz, hi xyz
-    const BvhNode *bvh;
+    const BvhNode *bvh;  // P nodes: 0 (the root's box), 1 .. P - 1 (internal)
+    const int *bvh_leaf; // P leaves
+    int bvh_leaves;      // P
     int n_tri;
     int const_normal; // every triangle has a constant interpolated normal
 };

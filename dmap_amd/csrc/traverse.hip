@@ -48,7 +48,10 @@ constexpr int KPAD = 8;        // ulps of padding per rounded interval
 constexpr int KNEWTON = 16;    // ulps of the residual's magnitude accepted as converged
 constexpr int SUBDIV_DEPTH = 5; // displaced_intersector.h:132
 constexpr int MAX_ITERS = 20;   // displaced_intersector.h:129
-constexpr int STACK = 16;       // 4 + 3 x 4 entries for a depth-5 search
+constexpr int STACK = 16;
+// BVHs of at most this many leaves go to LDS in single-object launches
+// (64 B per node + 4 B per leaf).
+constexpr int LDS_BVH_LEAVES = 128;       // 4 + 3 x 4 entries for a depth-5 search
 constexpr int BLOCK = 256;
 
 template <class R> __device__ inline R pad_of(R magnitude) { return (R)KPAD * Num<R>::eps * magnitude; }
@@ -906,64 +909,72 @@ __device__ inline bool ray_box(const R o[3], const R d[3], const R inv[3], const
 
 template <class R> __device__ inline R axis_of(const R v[3], int a) { return a == 0 ? v[0] : a == 1 ? v[1] : v[2]; }
 
-enum
-{
-    FROM_PARENT = 0,
-    FROM_SIBLING = 1,
-    FROM_CHILD = 2
-};
+// Traversal state of the heap-ordered BVH: `cur` is 0 before the first
+// call, a leaf after a call that returned its triangle, BVH_DONE after the
+// last; bit L of `trail` marks that the far child of the current path's
+// node at depth L is still to be visited.
+constexpr uint32_t BVH_DONE = 0xFFFFFFFFu;
 
-// Stackless traversal of the object's BVH (parent links, near child first
-// along the node's split axis): advances to the next triangle whose box
-// the ray meets within [t_min, t_max] and returns it, or -1 when the tree
-// is exhausted. The closest hit does not depend on the order triangles are
-// visited in (ObjectIntersector::intersect_flat keeps the nearest,
-// displaced_intersector.cpp:578-599); visiting near boxes first lets the
-// shrinking t_max cull the rest.
+__device__ __forceinline__ int depth_of(uint32_t node) { return 31 - __clz(node); }
+
+// Advances to the next triangle whose box the ray meets within [t_min,
+// t_max] and returns it, or -1 when the tree is exhausted: depth first, the
+// near child (by the ray's direction along the node's split axis) first, so
+// that a shrinking t_max culls the far boxes, which are tested again when
+// the walk comes back to them. The closest hit does not depend on the order
+// triangles are visited in (ObjectIntersector::intersect_flat keeps the
+// nearest, displaced_intersector.cpp:578-599).
 template <class R>
-__device__ int bvh_next(const BvhNode *N, int &cur, int &from, const R o[3], const R d[3], const R inv[3], R t_min,
-                        R t_max)
+__device__ __forceinline__ int bvh_next(const BvhNode *N, const int *leaf, uint32_t P, uint32_t &cur,
+                                        uint32_t &trail, const R o[3], const R d[3], const R inv[3], R t_min, R t_max)
 {
+    if (cur == BVH_DONE)
+        return -1;
+    bool pop = true;
+    if (cur == 0) {
+        if (!ray_box<R>(o, d, inv, N[0].lo[0], N[0].hi[0], t_min, t_max)) {
+            cur = BVH_DONE;
+            return -1;
+        }
+        cur = 1;
+        trail = 0;
+        pop = false;
+    }
     while (true) {
-        if (from == FROM_CHILD) {
-            if (cur == 0)
+        if (pop) {
+            if (trail == 0) {
+                cur = BVH_DONE;
                 return -1;
-            const BvhNode &par = N[N[cur].parent];
-            int near = axis_of(d, par.axis) >= (R)0 ? par.left : par.right;
-            if (cur == near) {
-                cur = near == par.left ? par.right : par.left;
-                from = FROM_SIBLING;
-            } else {
-                cur = N[cur].parent;
             }
+            int L = depth_of(trail);
+            trail &= ~(1u << L);
+            int dc = depth_of(cur);
+            uint32_t anc = cur >> (dc - L);
+            uint32_t far = (cur >> (dc - L - 1)) ^ 1u;
+            cur = far;
+            int c = (int)(far & 1u);
+            if (!ray_box<R>(o, d, inv, N[anc].lo[c], N[anc].hi[c], t_min, t_max))
+                continue;
+            pop = false;
+        }
+        if (cur >= P) {
+            int tri = leaf[cur - P];
+            if (tri >= 0)
+                return tri;
+            pop = true;
             continue;
         }
         const BvhNode &nd = N[cur];
-        if (!ray_box<R>(o, d, inv, nd.lo, nd.hi, t_min, t_max)) {
-            if (cur == 0)
-                return -1;
-            if (from == FROM_PARENT) {
-                const BvhNode &par = N[nd.parent];
-                cur = cur == par.left ? par.right : par.left;
-                from = FROM_SIBLING;
-            } else {
-                from = FROM_CHILD;
-            }
-            continue;
+        bool h0 = !(nd.empty & 1) && ray_box<R>(o, d, inv, nd.lo[0], nd.hi[0], t_min, t_max);
+        bool h1 = !(nd.empty & 2) && ray_box<R>(o, d, inv, nd.lo[1], nd.hi[1], t_min, t_max);
+        if (h0 && h1) {
+            trail |= 1u << depth_of(cur);
+            cur = 2 * cur + (axis_of(d, nd.axis) >= (R)0 ? 0u : 1u);
+        } else if (h0 || h1) {
+            cur = 2 * cur + (h1 ? 1u : 0u);
+        } else {
+            pop = true;
         }
-        if (nd.left < 0) {
-            int tri = ~nd.left;
-            if (from == FROM_PARENT && cur != 0) {
-                const BvhNode &par = N[nd.parent];
-                cur = cur == par.left ? par.right : par.left;
-                from = FROM_SIBLING;
-            } else {
-                from = FROM_CHILD;
-            }
-            return tri;
-        }
-        cur = axis_of(d, nd.axis) >= (R)0 ? nd.left : nd.right;
-        from = FROM_PARENT;
     }
 }
 
@@ -1100,6 +1111,24 @@ k_trace(TraceParams p)
     extern __shared__ __align__(16) unsigned char s_lds[];
     typename StackSlot<R>::T *s_t = (typename StackSlot<R>::T *)s_lds;
     uint16_t *s_code = (uint16_t *)(s_t + STACK * blockDim.x);
+    // Single-object fp32 launches over a small BVH keep a copy of it in LDS,
+    // after the stacks: the walk's dependent node reads are the bulk of the
+    // per-triangle bookkeeping. (The fp64 stacks already fill the LDS of
+    // four workgroups per CU.)
+    const bool lds_bvh = !MULTI && sizeof(R) == sizeof(float) && p.obj.bvh_leaves <= LDS_BVH_LEAVES;
+    unsigned char *bvh_lds = (unsigned char *)(s_code + STACK * blockDim.x);
+    if (lds_bvh) {
+        uint32_t *dst = (uint32_t *)bvh_lds;
+        const uint32_t *src = (const uint32_t *)p.obj.bvh;
+        const int node_words = p.obj.bvh_leaves * (int)(sizeof(BvhNode) / 4);
+        for (int q = threadIdx.x; q < node_words; q += blockDim.x)
+            dst[q] = src[q];
+        const uint32_t *lsrc = (const uint32_t *)p.obj.bvh_leaf;
+        uint32_t *ldst = (uint32_t *)(bvh_lds + sizeof(BvhNode) * LDS_BVH_LEAVES);
+        for (int q = threadIdx.x; q < p.obj.bvh_leaves; q += blockDim.x)
+            ldst[q] = lsrc[q];
+        __syncthreads();
+    }
 
     const int lane = threadIdx.x & 63;
     // Persistent waves: a lane that has finished its ray writes the result
@@ -1120,7 +1149,7 @@ k_trace(TraceParams p)
     R o_f[3] = {0, 0, 0}, d_f[3] = {0, 0, 1}, inv_f[3] = {0, 0, 0};
     int state = ST_DONE;
     bool walking = false;
-    int bvh_cur = 0, bvh_from = FROM_PARENT;
+    uint32_t bvh_cur = 0, bvh_trail = 0;
     bool single_done = false;
     int k = -1;
     int hit_tri = -1, hit_i = 0, hit_j = 0;
@@ -1219,7 +1248,7 @@ k_trace(TraceParams p)
                         state = ok ? ST_NEED : ST_DONE;
                         walking = false;
                         bvh_cur = 0;
-                        bvh_from = FROM_PARENT;
+                        bvh_trail = 0;
                         single_done = false;
                         k = -1;
                         hit_tri = -1;
@@ -1276,7 +1305,12 @@ k_trace(TraceParams p)
                         k = p.prim_mask;
                 }
             } else {
-                k = bvh_next<R>(DMAP_OBJ.bvh, bvh_cur, bvh_from, o_f, d_f, inv_f, t_min, t_best);
+                if (!MULTI && lds_bvh)
+                    k = bvh_next<R>((const BvhNode *)bvh_lds, (const int *)(bvh_lds + sizeof(BvhNode) * LDS_BVH_LEAVES),
+                                    (uint32_t)p.obj.bvh_leaves, bvh_cur, bvh_trail, o_f, d_f, inv_f, t_min, t_best);
+                else
+                    k = bvh_next<R>(DMAP_OBJ.bvh, DMAP_OBJ.bvh_leaf, (uint32_t)DMAP_OBJ.bvh_leaves, bvh_cur,
+                                    bvh_trail, o_f, d_f, inv_f, t_min, t_best);
             }
             if (k < 0) {
                 state = ST_DONE;
@@ -1860,6 +1894,14 @@ template <class R> static size_t trace_lds_bytes(unsigned block)
     return (size_t)STACK * block * (sizeof(typename StackSlot<R>::T) + sizeof(uint16_t));
 }
 
+// The LDS copy of a small BVH in single-object launches (k_trace's lds_bvh).
+template <class R> static size_t bvh_lds_bytes(const ObjectDev &obj)
+{
+    return sizeof(R) == sizeof(float) && obj.bvh_leaves <= LDS_BVH_LEAVES
+               ? (sizeof(BvhNode) + sizeof(int)) * LDS_BVH_LEAVES
+               : 0;
+}
+
 // Persistent single-object launches: workgroups per CU.
 constexpr int TRACE_GROUPS_PER_CU = 4;
 
@@ -1872,10 +1914,10 @@ template <class R, bool STATS, bool TWO>
 static void launch_trace_l(const TraceParams &p, unsigned blocks, hipStream_t stream)
 {
     if (p.obj.const_normal)
-        hipLaunchKernelGGL((k_trace<R, STATS, true, TWO, false>), dim3(blocks), dim3(BLOCK), trace_lds_bytes<R>(BLOCK),
+        hipLaunchKernelGGL((k_trace<R, STATS, true, TWO, false>), dim3(blocks), dim3(BLOCK), trace_lds_bytes<R>(BLOCK) + bvh_lds_bytes<R>(p.obj),
                            stream, p);
     else
-        hipLaunchKernelGGL((k_trace<R, STATS, false, TWO, false>), dim3(blocks), dim3(BLOCK), trace_lds_bytes<R>(BLOCK),
+        hipLaunchKernelGGL((k_trace<R, STATS, false, TWO, false>), dim3(blocks), dim3(BLOCK), trace_lds_bytes<R>(BLOCK) + bvh_lds_bytes<R>(p.obj),
                            stream, p);
 }
 
