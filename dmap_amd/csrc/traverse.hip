@@ -1264,17 +1264,19 @@ k_trace(TraceParams p)
         // Bookkeeping, per lane, up to its next block of node tests or leaf.
         while (state == ST_NEED) {
             if (walking) {
-                uint32_t m = (uint32_t)(trail >> (4 * lvl)) & 15u;
-                if (m == 0) {
-                    if (lvl >= root_level) {
-                        walking = false;
-                        continue;
-                    }
-                    ++lvl;
-                    pi >>= 1;
-                    pj >>= 1;
+                // Climb straight to the nearest level with cells left (no
+                // trail bits sit above the root block's level); none left
+                // ends this triangle's walk.
+                uint64_t rest = trail >> (4 * lvl);
+                if (rest == 0) {
+                    walking = false;
                     continue;
                 }
+                int up = (__ffsll((unsigned long long)rest) - 1) >> 2;
+                lvl += up;
+                pi >>= up;
+                pj >>= up;
+                uint32_t m = (uint32_t)(trail >> (4 * lvl)) & 15u;
                 int kk = __ffs(m) - 1;
                 trail &= ~((uint64_t)1 << (4 * lvl + kk));
                 int io = (kk >> 1) ^ wk.si, jo = (kk & 1) ^ wk.sj;
