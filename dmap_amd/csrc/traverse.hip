@@ -274,7 +274,7 @@ template <class R, bool CN>
 __device__ bool cell_test(const TraceParams &p, const PyramidView &P, const TriRecord &T, const Walk<R> &wk, int level,
                           int ci, int cj, float2 mm, float4 tp, R t_min, R t_max)
 {
-    R w = ldexp((R)1, level) / (R)P.n_leaf;
+    R w = ldexp((R)1, level - (__ffs(P.n_leaf) - 1)); // 2^level / n_leaf, n_leaf a power of two
     R clo_x = (R)ci * w, clo_y = (R)cj * w;
     // The cell clipped to the triangle's texture bounding box (node_bounds.cpp:94-99).
     R rlo_x = fmax(clo_x, (R)T.dlo[0]), rlo_y = fmax(clo_y, (R)T.dlo[1]);
@@ -1050,7 +1050,7 @@ __device__ uint32_t expand(const TraceParams &p, const PyramidView &P, const Tri
             g.tp[3] = fetch_tp(P, level, bi + 1, bj + 1);
         }
     }
-    R w = ldexp((R)1, level) / (R)P.n_leaf;
+    R w = ldexp((R)1, level - (__ffs(P.n_leaf) - 1)); // 2^level / n_leaf, n_leaf a power of two
     uint32_t mask = 0;
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
