@@ -474,12 +474,12 @@ static void tile_free(dmap_tile *t)
 {
     if (!t)
         return;
-    hipFree(t->texels);
-    hipFree(t->nodes);
-    hipFree(t->mm);
-    hipFree(t->tp);
-    hipFree(t->rr);
-    hipFree(t->scratch);
+    (void)hipFree(t->texels);
+    (void)hipFree(t->nodes);
+    (void)hipFree(t->mm);
+    (void)hipFree(t->tp);
+    (void)hipFree(t->rr);
+    (void)hipFree(t->scratch);
     delete t;
 }
 
@@ -634,7 +634,7 @@ int dmap_tile_read_nodes(const dmap_tile *tile, float *out)
 void dmap_tile_destroy(dmap_tile *tile)
 {
     if (tile)
-        hipSetDevice(tile->ctx->device);
+        (void)hipSetDevice(tile->ctx->device);
     tile_free(tile);
 }
 
@@ -661,10 +661,10 @@ static void object_free(dmap_object *o)
 {
     if (!o)
         return;
-    hipFree(o->tris);
-    hipFree(o->boxes);
-    hipFree(o->bvh);
-    hipFree(o->view_dev);
+    (void)hipFree(o->tris);
+    (void)hipFree(o->boxes);
+    (void)hipFree(o->bvh);
+    (void)hipFree(o->view_dev);
     delete o;
 }
 
@@ -686,7 +686,7 @@ static int object_bounds(dmap_object *o, const std::vector<HostTri> &tris)
     o->h_range_host.resize(2 * (size_t)n_tri);
     if (e == hipSuccess)
         e = hipMemcpy(o->h_range_host.data(), h_range, sizeof(float) * 2 * n_tri, hipMemcpyDeviceToHost);
-    hipFree(h_range);
+    (void)hipFree(h_range);
     if (e != hipSuccess)
         return fail(DMAP_ERR_HIP, "object bounds: %s", hipGetErrorString(e));
     o->boxes_host.resize(6 * (size_t)n_tri);
@@ -831,7 +831,7 @@ int dmap_object_triangle_info(const dmap_object *obj, int k, int *root_level, in
 void dmap_object_destroy(dmap_object *obj)
 {
     if (obj)
-        hipSetDevice(obj->ctx->device);
+        (void)hipSetDevice(obj->ctx->device);
     object_free(obj);
 }
 
@@ -901,7 +901,7 @@ static int trace(const dmap_object *obj, const dmap_rays *rays, int64_t n, const
         uint32_t *perm = nullptr;
         hipError_t e = reorder_rays(p.rays, n, obj->bound_lo, obj->bound_hi, (char *)ws + 256, rbytes, &perm, s);
         if (e != hipSuccess) {
-            hipFreeAsync(ws, s);
+            (void)hipFreeAsync(ws, s);
             return fail(DMAP_ERR_HIP, "intersect: ray reorder: %s", hipGetErrorString(e));
         }
         p.perm = perm;
@@ -914,7 +914,7 @@ static int trace(const dmap_object *obj, const dmap_rays *rays, int64_t n, const
     }
     hipError_t e = launch_trace(p, o.precision, s, ev[1] ? ev[0] : nullptr, ev[1]);
     if (ws)
-        hipFreeAsync(ws, s);
+        (void)hipFreeAsync(ws, s);
     HIP_TRY(e);
     return DMAP_OK;
 }
@@ -1007,7 +1007,7 @@ int dmap_intersect_adjoint(const dmap_object *obj, const dmap_rays *rays, int64_
         if (e != hipSuccess)
             rc = fail(DMAP_ERR_HIP, "dmap_intersect_adjoint: %s", hipGetErrorString(e));
     }
-    hipFreeAsync(ws, s);
+    (void)hipFreeAsync(ws, s);
     return rc;
 }
 

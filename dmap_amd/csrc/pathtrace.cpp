@@ -396,7 +396,7 @@ int dmap_pathtrace(const dmap_object *obj, const double *cam_o, const double *ca
     uint32_t *pinned = nullptr;
     int rc = DMAP_OK;
     if (cap > 0 && hipHostMalloc((void **)&pinned, 2 * sizeof(uint32_t), hipHostMallocDefault) != hipSuccess) {
-        hipFreeAsync(ws, s);
+        (void)hipFreeAsync(ws, s);
         return fail(DMAP_ERR_MEMORY, "dmap_pathtrace: pinned counters");
     }
     for (int64_t first = 0; first < n_paths && rc == DMAP_OK; first += cap) {
@@ -461,10 +461,10 @@ int dmap_pathtrace(const dmap_object *obj, const double *cam_o, const double *ca
         hipLaunchKernelGGL(k_pt_scale, dim3(grid_of(n_pix)), dim3(BLOCK), 0, s, n_pix, 1.0 / spp, image);
     hipError_t e = hipGetLastError();
     if (ws)
-        hipFreeAsync(ws, s);
+        (void)hipFreeAsync(ws, s);
     if (pinned) {
-        hipStreamSynchronize(s);
-        hipHostFree(pinned);
+        (void)hipStreamSynchronize(s);
+        (void)hipHostFree(pinned);
     }
     if (rc != DMAP_OK)
         return rc;

@@ -48,7 +48,7 @@ __global__ void k_ray_keys(RaysDev rays, int64_t n, float lx, float ly, float lz
 size_t reorder_workspace_bytes(int64_t n)
 {
     size_t temp = 0;
-    hipcub::DeviceRadixSort::SortPairs(nullptr, temp, (const uint32_t *)nullptr, (uint32_t *)nullptr,
+    (void)hipcub::DeviceRadixSort::SortPairs(nullptr, temp, (const uint32_t *)nullptr, (uint32_t *)nullptr,
                                        (const uint32_t *)nullptr, (uint32_t *)nullptr, (int)n, 0, 30);
     return 4 * (size_t)n * 4 + temp + 256;
 }

@@ -323,7 +323,7 @@ int dmap_server_create(dmap_ctx *ctx, int n_slots, int64_t step_rays, int64_t hi
             if (e == hipSuccess)
                 e = hipGraphInstantiate(&sl->graph[kind], g, nullptr, nullptr, 0);
             if (g)
-                hipGraphDestroy(g);
+                (void)hipGraphDestroy(g);
         }
         if (e != hipSuccess)
             break;
@@ -452,7 +452,7 @@ void dmap_server_destroy(dmap_server *S)
 {
     if (!S)
         return;
-    hipSetDevice(S->ctx->device);
+    (void)hipSetDevice(S->ctx->device);
     for (Slot *sl : S->slots) {
         if (sl->stream)
             (void)hipStreamSynchronize(sl->stream);
