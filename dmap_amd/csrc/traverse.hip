@@ -26,6 +26,8 @@
 // in texel-local coordinates (a, b) in [0, 1]^2, where its terms are small.
 #include <hip/hip_runtime.h>
 
+#include <atomic>
+
 #include "kernels.h"
 #include "layout.h"
 
@@ -1904,23 +1906,53 @@ template <class R> static size_t bvh_lds_bytes(const ObjectDev &obj)
                : 0;
 }
 
-// Persistent single-object launches: workgroups per CU.
-constexpr int TRACE_GROUPS_PER_CU = 4;
+// Persistent single-object launches: workgroups per CU when the runtime
+// cannot say how many stay resident.
+constexpr int TRACE_GROUPS_PER_CU = 3;
 
 // Multi-request launches use one-wave workgroups: a workgroup's resources
 // free as soon as its wave is done, so the next step's waves fill the tail
 // of a running step wave by wave.
 constexpr unsigned MULTI_BLOCK = 64;
 
+static int cu_count();
+
+// Workgroups of a single-object trace kernel that fit on one CU at once
+// (registers and LDS), asked of the runtime once per kernel and LDS size.
+template <class R, bool STATS, bool CN, bool TWO> static int resident_groups(size_t lds, bool with_bvh)
+{
+    static std::atomic<int> cached[2];
+    int n = cached[with_bvh ? 1 : 0].load(std::memory_order_relaxed);
+    if (n <= 0) {
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_trace<R, STATS, CN, TWO, false>, BLOCK, lds) !=
+                hipSuccess ||
+            n <= 0)
+            n = TRACE_GROUPS_PER_CU;
+        cached[with_bvh ? 1 : 0].store(n, std::memory_order_relaxed);
+    }
+    return n;
+}
+
+// Persistent waves: as many workgroups as stay resident together on every
+// CU (a workgroup beyond that would start only when another ends, and the
+// rays it took from the queue would finish last), fewer for small launches.
+template <class R, bool STATS, bool CN, bool TWO>
+static void launch_trace_k(const TraceParams &p, int64_t want, hipStream_t stream)
+{
+    const size_t bvh_lds = bvh_lds_bytes<R>(p.obj);
+    const size_t lds = trace_lds_bytes<R>(BLOCK) + bvh_lds;
+    const int64_t cap = (int64_t)cu_count() * resident_groups<R, STATS, CN, TWO>(lds, bvh_lds > 0);
+    const unsigned blocks = (unsigned)(want < cap ? want : cap);
+    hipLaunchKernelGGL((k_trace<R, STATS, CN, TWO, false>), dim3(blocks), dim3(BLOCK), lds, stream, p);
+}
+
 template <class R, bool STATS, bool TWO>
-static void launch_trace_l(const TraceParams &p, unsigned blocks, hipStream_t stream)
+static void launch_trace_l(const TraceParams &p, int64_t want, hipStream_t stream)
 {
     if (p.obj.const_normal)
-        hipLaunchKernelGGL((k_trace<R, STATS, true, TWO, false>), dim3(blocks), dim3(BLOCK), trace_lds_bytes<R>(BLOCK) + bvh_lds_bytes<R>(p.obj),
-                           stream, p);
+        launch_trace_k<R, STATS, true, TWO>(p, want, stream);
     else
-        hipLaunchKernelGGL((k_trace<R, STATS, false, TWO, false>), dim3(blocks), dim3(BLOCK), trace_lds_bytes<R>(BLOCK) + bvh_lds_bytes<R>(p.obj),
-                           stream, p);
+        launch_trace_k<R, STATS, false, TWO>(p, want, stream);
 }
 
 static int cu_count()
@@ -1936,12 +1968,12 @@ static int cu_count()
 }
 
 template <class R, bool STATS>
-static void launch_trace_r(const TraceParams &p, unsigned blocks, hipStream_t stream)
+static void launch_trace_r(const TraceParams &p, int64_t want, hipStream_t stream)
 {
     if (p.leaf == 1)
-        launch_trace_l<R, STATS, true>(p, blocks, stream);
+        launch_trace_l<R, STATS, true>(p, want, stream);
     else
-        launch_trace_l<R, STATS, false>(p, blocks, stream);
+        launch_trace_l<R, STATS, false>(p, want, stream);
 }
 
 hipError_t launch_trace(const TraceParams &p, int precision, hipStream_t stream, hipEvent_t ev_start,
@@ -1949,10 +1981,7 @@ hipError_t launch_trace(const TraceParams &p, int precision, hipStream_t stream,
 {
     if (p.n_rays <= 0)
         return hipSuccess;
-    // Persistent waves: enough workgroups to fill every CU a few times over.
-    int cus = cu_count();
-    int64_t want = (p.n_rays + BLOCK - 1) / BLOCK;
-    unsigned blocks = (unsigned)(want < (int64_t)cus * TRACE_GROUPS_PER_CU ? want : (int64_t)cus * TRACE_GROUPS_PER_CU);
+    const int64_t want = (p.n_rays + BLOCK - 1) / BLOCK;
     hipLaunchKernelGGL(k_queue_reset, dim3(1), dim3(1), 0, stream, p.queue);
     bool stats = p.stats.node_tests || p.stats.leaf_tests || p.stats.sub_squares || p.stats.triangle_tests ||
                  p.stats.newton_runs || p.stats.newton_iters;
@@ -1960,14 +1989,14 @@ hipError_t launch_trace(const TraceParams &p, int precision, hipStream_t stream,
         (void)hipEventRecord(ev_start, stream);
     if (precision == 1) {
         if (stats)
-            launch_trace_r<double, true>(p, blocks, stream);
+            launch_trace_r<double, true>(p, want, stream);
         else
-            launch_trace_r<double, false>(p, blocks, stream);
+            launch_trace_r<double, false>(p, want, stream);
     } else {
         if (stats)
-            launch_trace_r<float, true>(p, blocks, stream);
+            launch_trace_r<float, true>(p, want, stream);
         else
-            launch_trace_r<float, false>(p, blocks, stream);
+            launch_trace_r<float, false>(p, want, stream);
     }
     hipError_t e = hipGetLastError();
     if (ev_end)
