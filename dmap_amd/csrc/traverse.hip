@@ -240,6 +240,43 @@ template <> __device__ inline bool square_outside<double>(const TriRecord &T, do
     return false;
 }
 
+// The square lies inside the triangle's domain with at least
+// square_outside's rounding room to spare on every side, so square_outside
+// is false for every square within it: a node group whose 2 x 2 block passes
+// this skips its four children's tests.
+template <class R> __device__ inline bool square_inside(const TriRecord &T, R cx, R cy, R s)
+{
+    R m = pad_of<R>((R)1 + fabs(cx) + fabs(cy) + s);
+    if (!(cx - s >= (R)T.dlo[0] + m && cx + s <= (R)T.dhi[0] - m && cy - s >= (R)T.dlo[1] + m &&
+          cy + s <= (R)T.dhi[1] - m))
+        return false;
+    for (int k = 0; k < 3; ++k) {
+        R nx = (R)T.dn[2 * k], ny = (R)T.dn[2 * k + 1];
+        R proj = nx * cx + ny * cy;
+        R ext = (fabs(nx) + fabs(ny)) * s;
+        R mk = pad_of<R>(fabs(nx * cx) + fabs(ny * cy) + ext + fabs((R)T.dd[k]));
+        if (!(proj - ext >= (R)T.dd[k] + mk))
+            return false;
+    }
+    return true;
+}
+template <> __device__ inline bool square_inside<double>(const TriRecord &T, double cx, double cy, double s)
+{
+    double m = pad_of<double>(1.0 + fabs(cx) + fabs(cy) + s);
+    if (!(cx - s >= T.dlo_d[0] + m && cx + s <= T.dhi_d[0] - m && cy - s >= T.dlo_d[1] + m &&
+          cy + s <= T.dhi_d[1] - m))
+        return false;
+    for (int k = 0; k < 3; ++k) {
+        double nx = T.dn_d[2 * k], ny = T.dn_d[2 * k + 1];
+        double proj = nx * cx + ny * cy;
+        double ext = (fabs(nx) + fabs(ny)) * s;
+        double mk = pad_of<double>(fabs(nx * cx) + fabs(ny * cy) + ext + fabs(T.dd_d[k]));
+        if (!(proj - ext >= T.dd_d[k] + mk))
+            return false;
+    }
+    return true;
+}
+
 // UvTriangle::inside (uv_clip.h:14-20).
 template <class R> __device__ inline bool domain_inside(const TriRecord &T, R s, R t)
 {
@@ -1053,6 +1090,7 @@ __device__ uint32_t expand(const TraceParams &p, const PyramidView &P, const Tri
         }
     }
     R w = ldexp((R)1, level - (__ffs(P.n_leaf) - 1)); // 2^level / n_leaf, n_leaf a power of two
+    const bool inside = square_inside<R>(T, ((R)bi + (R)1) * w, ((R)bj + (R)1) * w, w);
     uint32_t mask = 0;
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
@@ -1060,7 +1098,7 @@ __device__ uint32_t expand(const TraceParams &p, const PyramidView &P, const Tri
         int ci = bi + io, cj = bj + jo;
         if (!((valid >> q) & 1u))
             continue;
-        if (square_outside<R>(T, ((R)ci + (R)0.5) * w, ((R)cj + (R)0.5) * w, (R)0.5 * w))
+        if (!inside && square_outside<R>(T, ((R)ci + (R)0.5) * w, ((R)cj + (R)0.5) * w, (R)0.5 * w))
             continue;
         ++node_tests;
         float4 tp = slab ? g.tp[q] : make_float4(0.f, 0.f, 0.f, 0.f);
