@@ -322,6 +322,47 @@ class ObjectIntersector
         return occ.download();
     }
 
+    // Wavefront path tracing of this object (dmap_pathtrace): camera rays
+    // as host queries (origin, direction per pixel), the light as corner,
+    // edges and the unit normal of its emitting side. Returns the mean
+    // radiance per pixel; ray counts go to *closest_rays / *shadow_rays.
+    struct PathScene
+    {
+        double light_corner[3] = {3, 3, 3}, light_u[3] = {0, 2, 0}, light_v[3] = {2, 0, 0}, light_n[3] = {0, 0, -1};
+        double radiance = 10.0, env = 0.2, albedo = 0.7;
+    };
+    std::vector<double> pathtrace(const std::vector<RayQuery> &camera, int spp, int depth, const PathScene &scene,
+                                  uint32_t seed = 0, bool mis = true, const TraversalOptions &options = {},
+                                  int64_t *closest_rays = nullptr, int64_t *shadow_rays = nullptr) const
+    {
+        const size_t n = camera.size();
+        if (n == 0)
+            return {};
+        std::vector<double> o(3 * n), d(3 * n);
+        for (size_t r = 0; r < n; ++r)
+            for (int q = 0; q < 3; ++q) {
+                o[q * n + r] = camera[r].o[q];
+                d[q * n + r] = camera[r].d[q];
+            }
+        DeviceArray<double> o_dev(*ctx_, o), d_dev(*ctx_, d), image(*ctx_, n);
+        dmap_pt_scene sc;
+        for (int q = 0; q < 3; ++q) {
+            sc.light_corner[q] = scene.light_corner[q];
+            sc.light_u[q] = scene.light_u[q];
+            sc.light_v[q] = scene.light_v[q];
+            sc.light_n[q] = scene.light_n[q];
+        }
+        sc.radiance = scene.radiance;
+        sc.env = scene.env;
+        sc.albedo = scene.albedo;
+        dmap_options opt = options.c();
+        check(dmap_pathtrace(h_, o_dev.data(), d_dev.data(), (int64_t)n, spp, depth, seed, mis ? 0 : 1, &sc, &opt,
+                             image.data(), closest_rays, shadow_rays, nullptr),
+              "dmap_pathtrace");
+        check(dmap_stream_synchronize(nullptr), "dmap_stream_synchronize");
+        return image.download();
+    }
+
     int n_tri = 0;
 
   private:

@@ -15,6 +15,9 @@
 //      reference's Newton leaf (displaced_intersector.cpp:239-323) converges
 //      to one of them; the double oracle then reports the same exit root.
 //  (5) the both test costs no more node tests than the box test, summed.
+//  (6) path tracing through the C++ API: a white furnace (albedo 1,
+//      environment 1, no light) returns at most 1 per pixel and about 1 on
+//      average, and the ray counts add up.
 // Prints the counts and "validate_traversal_gpu: PASS" or "FAIL"; exit 0 on
 // PASS.
 #include <algorithm>
@@ -212,6 +215,37 @@ int main()
                         "%lld grazing past it\n",
                         aimed.size(), (long long)miss, (long long)beyond, (long long)grazing);
             pass = pass && miss == 0 && beyond == 0;
+        }
+        // (6) white furnace through the wavefront path tracer
+        {
+            ObjectIntersector obj(ctx, tile, flat_quad());
+            std::vector<RayQuery> cam(4096);
+            std::uniform_real_distribution<double> U(0.0, 1.0);
+            for (RayQuery &q : cam) {
+                double o[3] = {0.5, -0.3, 0.8}, tgt[3] = {0.1 + 0.8 * U(rng), 0.1 + 0.8 * U(rng), 0.0};
+                for (int k = 0; k < 3; ++k) {
+                    q.o[k] = o[k];
+                    q.d[k] = tgt[k] - o[k];
+                }
+            }
+            ObjectIntersector::PathScene white;
+            white.radiance = 0.0;
+            white.env = 1.0;
+            white.albedo = 1.0;
+            for (int k = 0; k < 3; ++k)
+                white.light_corner[k] = 1e3;
+            int64_t n_closest = 0, n_shadow = 0;
+            std::vector<double> img = obj.pathtrace(cam, 4, 16, white, 5, false, TraversalOptions{}, &n_closest,
+                                                    &n_shadow);
+            double mx = 0, mean = 0;
+            for (double v : img) {
+                mx = std::max(mx, v);
+                mean += v / img.size();
+            }
+            std::printf("(6) white furnace: %zu pixels x 4 spp, %lld closest + %lld shadow rays, max %.6f mean %.6f\n",
+                        img.size(), (long long)n_closest, (long long)n_shadow, mx, mean);
+            pass = pass && mx <= 1.0 + 1e-12 && mean > 0.99 && n_closest >= (int64_t)(4 * cam.size()) &&
+                   n_shadow == 0;
         }
     } catch (const Error &e) {
         std::printf("error: %s\n", e.what());
