@@ -381,6 +381,14 @@ int dmap_pathtrace(const dmap_object *obj, const double *cam_o, const double *ca
     L.albedo = scene->albedo;
     if (!(L.area > 0))
         return fail(DMAP_ERR_INVALID, "dmap_pathtrace: the light's edges span no area");
+    // The queues stay in pixel order (camera rays by pixel, later bounces
+    // compacted wave by wave in that order), which is coherent already:
+    // sorting each batch cost more than it saved (20.4 against 17.2 ms for
+    // 1024^2 x 4 spp on the 4096^2 plane), so the calls below do not.
+    dmap_options opt{2, 1, 0, 0, 0};
+    if (options)
+        opt = *options;
+    opt.reorder = 0;
     HIP_TRY(hipSetDevice(obj->ctx->device));
     hipStream_t s = (hipStream_t)stream;
     int64_t cnt_closest = 0, cnt_shadow = 0;
@@ -409,7 +417,7 @@ int dmap_pathtrace(const dmap_object *obj, const double *cam_o, const double *ca
             dmap_rays R = as_rays(W.rays);
             dmap_hits H{W.hit_f[0], W.hit_f[1], W.hit_f[2], nullptr, nullptr, W.hit_f[5], W.hit_f[6], W.hit_f[7],
                         W.hit_i[0], W.hit_i[1], W.hit_i[2]};
-            if ((rc = dmap_intersect_closest(obj, &R, live, &H, -1, options, nullptr, stream)) != DMAP_OK)
+            if ((rc = dmap_intersect_closest(obj, &R, live, &H, -1, &opt, nullptr, stream)) != DMAP_OK)
                 break;
             cnt_closest += live;
             bool last = b == depth;
@@ -447,7 +455,7 @@ int dmap_pathtrace(const dmap_object *obj, const double *cam_o, const double *ca
             int64_t n_next = pinned[0], n_shadow = pinned[1];
             if (n_shadow > 0) {
                 dmap_rays SR = as_rays(W.shadow);
-                if ((rc = dmap_intersect_any(obj, &SR, n_shadow, W.occluded, -1, options, stream)) != DMAP_OK)
+                if ((rc = dmap_intersect_any(obj, &SR, n_shadow, W.occluded, -1, &opt, stream)) != DMAP_OK)
                     break;
                 cnt_shadow += n_shadow;
                 hipLaunchKernelGGL(k_pt_shadow, dim3(grid_of(n_shadow)), dim3(BLOCK), 0, s, n_shadow, W.occluded,

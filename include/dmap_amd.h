@@ -16,8 +16,9 @@
  *   dmap_intersect_any    the occluded callback (displaced_geometry.cpp:79-106)
  *   dmap_hit_record       DisplacedGeometry::surface_point + compute_intersection's
  *                         frame (displaced_geometry.cpp:126-183)
- * The adjoint, tile texel updates and the serving loop have no counterpart
- * in the reference (see their declarations).
+ * The adjoint, tile texel updates, the serving loop, the wavefront path
+ * tracing call, kernel timing and the device-memory helpers have no
+ * counterpart in the reference (see their declarations).
  *
  * Conventions (as the reference's, displaced_intersector.h:107-114):
  *   rays are in object space, d need not be unit length, t is in units of
@@ -127,9 +128,9 @@ int dmap_tile_read_level(const dmap_tile *tile, int level, float *out);
 int dmap_tile_read_nodes(const dmap_tile *tile, float *out);
 void dmap_tile_destroy(dmap_tile *tile);
 
-/* An object: n_tri base triangles over one tile. positions, normals: 9
- * doubles per triangle (p0 p1 p2, m0 m1 m2); uvs: 6 doubles per triangle
- * (t0 t1 t2). Host pointers. */
+/* An object: n_tri (1 .. 2^30) base triangles over one tile. positions,
+ * normals: 9 doubles per triangle (p0 p1 p2, m0 m1 m2); uvs: 6 doubles per
+ * triangle (t0 t1 t2). Host pointers. */
 int dmap_object_create(dmap_ctx *ctx, dmap_tile *tile, int n_tri, const double *positions, const double *normals,
                        const double *uvs, dmap_object **out);
 /* The top-level BVH over the triangles' AABBs stands in for the reference's
@@ -228,9 +229,11 @@ void dmap_server_destroy(dmap_server *server);
  * random numbers with `seed`. cam_o, cam_d: device, 3 rows of n_pix doubles
  * (directions need not be unit). image: device, n_pix doubles, set to the
  * mean radiance per pixel. The counts of closest-hit and any-hit rays
- * traced go to *closest_rays and *shadow_rays when not NULL. Returns when
- * the image is complete on `stream`'s timeline (it synchronizes with the
- * stream once per bounce to size the next batch). */
+ * traced go to *closest_rays and *shadow_rays when not NULL. options as
+ * dmap_intersect_closest's, except that reorder is ignored: the wavefront's
+ * queues stay in pixel order, which is coherent already. Returns when the
+ * image is complete on `stream`'s timeline (it synchronizes with the stream
+ * once per bounce to size the next batch). */
 typedef struct dmap_pt_scene
 {
     double light_corner[3], light_u[3], light_v[3], light_n[3];
