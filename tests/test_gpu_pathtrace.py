@@ -112,3 +112,20 @@ def test_pathtrace_device_mis_bsdf_and_furnace(torch_gpu):
     assert empty.numel() == 0
     direct = PT.render_device(scene, o, d, spp=1, depth=0).cpu().numpy()
     assert np.all((direct == 0.0) | (direct == scene.env) | (direct == scene.radiance))
+
+
+def test_pathtrace_device_chunks(torch_gpu):
+    """More paths than one wavefront holds (4,194,304): the device form runs
+    them in chunks and still matches the torch loop path for path."""
+    torch = torch_gpu
+    tile, obj, sc, cam_o, cam_d = _small(torch, res=560)
+    scene = PT.Scene(PT.GpuIntersector(obj))
+    o, d = cam_o.cuda(), cam_d.cuda()
+    spp = 14
+    assert o.shape[1] * spp > 1 << 22
+    st_d, st_t = {}, {}
+    img_d = PT.render_device(scene, o, d, spp=spp, depth=2, seed=5, stats=st_d).cpu().numpy()
+    img_t = PT.render(torch, scene, o, d, spp=spp, depth=2, seed=5, stats=st_t).cpu().numpy()
+    assert st_d == st_t, (st_d, st_t)
+    close = np.abs(img_d - img_t) <= 1e-9 * (1 + np.abs(img_t))
+    assert close.mean() > 0.995, close.mean()
