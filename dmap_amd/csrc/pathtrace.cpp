@@ -15,6 +15,10 @@
 //               compacted into fresh queues with one atomic per wave
 //   trace any-hit (the next-event rays), k_pt_shadow: add the unoccluded ones
 //
+// Each path sums its radiance in its own slot (one lane owns a path at a
+// time, so no atomics), and k_pt_gather averages a pixel's samples once the
+// chunk is done; chunks hold whole pixels.
+//
 // All shading arithmetic is in double, with contraction off, so that the
 // same paths through another intersector give the same image; rays go to
 // the intersector in fp32, as its ABI takes them. Random numbers are a
@@ -34,7 +38,7 @@ constexpr double LIGHT_EPS = 1e-4; // shadow rays stop this short of the light
 constexpr double SPAWN_EPS = 1e-4; // new rays start this far off the surface
 constexpr double PI = 3.141592653589793;
 constexpr int BLOCK = 256;
-// Paths per wavefront chunk: about 260 B of queues each.
+// Paths per wavefront chunk: about 270 B of queues each.
 constexpr int64_t MAX_CHUNK = (int64_t)1 << 22;
 
 struct Light
@@ -117,7 +121,7 @@ __device__ __forceinline__ void put_ray(const RayPlanes &R, int64_t k, const dou
 }
 
 __global__ void k_pt_camera(int64_t first, int64_t count, int spp, const double *cam_o, const double *cam_d,
-                            int64_t n_pix, PathQueue Q, RayPlanes R)
+                            int64_t n_pix, PathQueue Q, RayPlanes R, double *radiance)
 {
     int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
     if (i >= count)
@@ -138,6 +142,7 @@ __global__ void k_pt_camera(int64_t first, int64_t count, int spp, const double 
     Q.pid[i] = (uint32_t)pid;
     Q.beta[i] = 1.0;
     Q.pdf_prev[i] = 0.0;
+    radiance[i] = 0.0;
     put_ray(R, i, o, d, INFINITY);
 }
 
@@ -149,7 +154,8 @@ struct ShadeArgs
     double *shadow_contrib;
     uint32_t *shadow_pid;
     uint32_t *counters; // [0] next paths, [1] shadow rays
-    double *image;
+    double *radiance;   // per path of the chunk, indexed pid - first
+    int64_t first;
     Light L;
     int64_t n;
     int spp, bounce, last, mis;
@@ -167,7 +173,7 @@ __global__ void k_pt_shade(ShadeArgs a)
     const Light &L = a.L;
     if (live) {
         pid = a.in.pid[i];
-        int64_t pix = pid / (uint32_t)a.spp;
+        double &Lp = a.radiance[pid - a.first];
         beta = a.in.beta[i];
         double pdf_prev = a.in.pdf_prev[i];
         double o[3], d[3];
@@ -197,9 +203,9 @@ __global__ void k_pt_shade(ShadeArgs a)
                 double pdf_l = (t_light * t_light) / (L.area * fmax(cos_l, 1e-12));
                 w = pdf_prev * pdf_prev / (pdf_prev * pdf_prev + pdf_l * pdf_l);
             }
-            unsafeAtomicAdd(a.image + pix, beta * w * L.radiance);
+            Lp += beta * w * L.radiance;
         } else if (!hit) {
-            unsafeAtomicAdd(a.image + pix, beta * L.env);
+            Lp += beta * L.env;
         } else if (!a.last) {
             double ng[3];
             for (int q = 0; q < 3; ++q)
@@ -273,27 +279,26 @@ __global__ void k_pt_shade(ShadeArgs a)
     }
 }
 
-__global__ void k_pt_shadow(int64_t n, const uint8_t *occluded, const double *contrib, const uint32_t *pid, int spp,
-                            double *image)
+__global__ void k_pt_shadow(int64_t n, const uint8_t *occluded, const double *contrib, const uint32_t *pid,
+                            int64_t first, double *radiance)
 {
     int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
     if (i >= n || occluded[i])
         return;
-    unsafeAtomicAdd(image + pid[i] / (uint32_t)spp, contrib[i]);
+    radiance[pid[i] - first] += contrib[i];
 }
 
-__global__ void k_pt_scale(int64_t n, double s, double *image)
+// The mean over each pixel's samples: pixels first_pix .. first_pix + n - 1,
+// whose samples are consecutive paths of the chunk.
+__global__ void k_pt_gather(int64_t first_pix, int64_t n, int spp, const double *radiance, double *image)
 {
-    int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
-    if (i < n)
-        image[i] *= s;
-}
-
-__global__ void k_pt_zero(int64_t n, double *image)
-{
-    int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
-    if (i < n)
-        image[i] = 0.0;
+    int64_t p = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+    if (p >= n)
+        return;
+    double sum = 0.0;
+    for (int k = 0; k < spp; ++k)
+        sum += radiance[p * spp + k];
+    image[first_pix + p] = sum / spp;
 }
 
 unsigned grid_of(int64_t n) { return (unsigned)((n + BLOCK - 1) / BLOCK); }
@@ -309,6 +314,7 @@ struct Workspace
     double *contrib;
     uint32_t *shadow_pid;
     uint8_t *occluded;
+    double *radiance;
     uint32_t *counters;
 
     size_t carve(char *base, int64_t cap)
@@ -339,6 +345,7 @@ struct Workspace
         contrib = (double *)take(8);
         shadow_pid = (uint32_t *)take(4);
         occluded = (uint8_t *)take(1);
+        radiance = (double *)take(8);
         counters = base ? (uint32_t *)(base + off) : nullptr;
         return off + 256;
     }
@@ -392,9 +399,11 @@ int dmap_pathtrace(const dmap_object *obj, const double *cam_o, const double *ca
     HIP_TRY(hipSetDevice(obj->ctx->device));
     hipStream_t s = (hipStream_t)stream;
     int64_t cnt_closest = 0, cnt_shadow = 0;
-    if (n_pix > 0)
-        hipLaunchKernelGGL(k_pt_zero, dim3(grid_of(n_pix)), dim3(BLOCK), 0, s, n_pix, image);
-    const int64_t cap = n_paths < MAX_CHUNK ? n_paths : MAX_CHUNK;
+    // Chunks of whole pixels.
+    const int64_t cap = n_paths < MAX_CHUNK ? n_paths : (MAX_CHUNK / spp) * spp;
+    if (n_paths > 0 && cap <= 0)
+        return fail(DMAP_ERR_INVALID, "dmap_pathtrace: spp %d exceeds the %lld paths of one wavefront", spp,
+                    (long long)MAX_CHUNK);
     char *ws = nullptr;
     Workspace W;
     if (cap > 0)
@@ -410,7 +419,7 @@ int dmap_pathtrace(const dmap_object *obj, const double *cam_o, const double *ca
     for (int64_t first = 0; first < n_paths && rc == DMAP_OK; first += cap) {
         int64_t count = n_paths - first < cap ? n_paths - first : cap;
         hipLaunchKernelGGL(k_pt_camera, dim3(grid_of(count)), dim3(BLOCK), 0, s, first, count, spp, cam_o, cam_d,
-                           n_pix, W.q[0], W.rays);
+                           n_pix, W.q[0], W.rays, W.radiance);
         int cur = 0;
         int64_t live = count;
         for (int b = 0; b <= depth && live > 0; ++b) {
@@ -436,7 +445,8 @@ int dmap_pathtrace(const dmap_object *obj, const double *cam_o, const double *ca
             a.shadow_contrib = W.contrib;
             a.shadow_pid = W.shadow_pid;
             a.counters = W.counters;
-            a.image = image;
+            a.radiance = W.radiance;
+            a.first = first;
             a.L = L;
             a.n = live;
             a.spp = spp;
@@ -459,14 +469,15 @@ int dmap_pathtrace(const dmap_object *obj, const double *cam_o, const double *ca
                     break;
                 cnt_shadow += n_shadow;
                 hipLaunchKernelGGL(k_pt_shadow, dim3(grid_of(n_shadow)), dim3(BLOCK), 0, s, n_shadow, W.occluded,
-                                   W.contrib, W.shadow_pid, spp, image);
+                                   W.contrib, W.shadow_pid, first, W.radiance);
             }
             live = n_next;
             cur ^= 1;
         }
+        if (rc == DMAP_OK)
+            hipLaunchKernelGGL(k_pt_gather, dim3(grid_of(count / spp)), dim3(BLOCK), 0, s, first / spp, count / spp,
+                               spp, W.radiance, image);
     }
-    if (rc == DMAP_OK && n_pix > 0)
-        hipLaunchKernelGGL(k_pt_scale, dim3(grid_of(n_pix)), dim3(BLOCK), 0, s, n_pix, 1.0 / spp, image);
     hipError_t e = hipGetLastError();
     if (ws)
         (void)hipFreeAsync(ws, s);
