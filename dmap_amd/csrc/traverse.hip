@@ -50,10 +50,10 @@ constexpr int KPAD = 8;        // ulps of padding per rounded interval
 constexpr int KNEWTON = 16;    // ulps of the residual's magnitude accepted as converged
 constexpr int SUBDIV_DEPTH = 5; // displaced_intersector.h:132
 constexpr int MAX_ITERS = 20;   // displaced_intersector.h:129
-constexpr int STACK = 16;
+constexpr int STACK = 16;       // 4 + 3 x 4 entries for a depth-5 search
 // BVHs of at most this many leaves go to LDS in single-object launches
 // (64 B per node + 4 B per leaf).
-constexpr int LDS_BVH_LEAVES = 128;       // 4 + 3 x 4 entries for a depth-5 search
+constexpr int LDS_BVH_LEAVES = 128;
 constexpr int BLOCK = 256;
 
 template <class R> __device__ inline R pad_of(R magnitude) { return (R)KPAD * Num<R>::eps * magnitude; }
