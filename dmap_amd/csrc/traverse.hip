@@ -1137,12 +1137,15 @@ constexpr int MULTI_REFILL = 48;
 // from the BVH) to its next block of node tests, then runs those node
 // tests for the whole wave at once; pending leaves are searched together
 // once enough lanes have one.
-#ifndef DMAP_TRACE_WAVES
-#define DMAP_TRACE_WAVES 3
-#endif
+// Waves per SIMD the trace kernel is compiled for: 3 for fp32 objects with
+// constant normals (168 VGPRs; 2 and 4 measured slower); 2 for the
+// general-normal fp32 kernel (it spills 136 B per lane at 3, and runs 4%
+// faster at 2) and for fp64, whose doubled state would spill 400-1000 B per
+// lane at 3 and runs 1.9x faster at 2.
+template <class R, bool CN> constexpr int trace_waves() { return (sizeof(R) == sizeof(double) || !CN) ? 2 : 3; }
 
 template <class R, bool STATS, bool CN, bool TWO, bool MULTI>
-__global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(DMAP_TRACE_WAVES, 8)))
+__global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(trace_waves<R, CN>(), 8)))
 k_trace(TraceParams p)
 {
 
@@ -1151,11 +1154,10 @@ k_trace(TraceParams p)
     extern __shared__ __align__(16) unsigned char s_lds[];
     typename StackSlot<R>::T *s_t = (typename StackSlot<R>::T *)s_lds;
     uint16_t *s_code = (uint16_t *)(s_t + STACK * blockDim.x);
-    // Single-object fp32 launches over a small BVH keep a copy of it in LDS,
+    // Single-object launches over a small BVH keep a copy of it in LDS,
     // after the stacks: the walk's dependent node reads are the bulk of the
-    // per-triangle bookkeeping. (The fp64 stacks already fill the LDS of
-    // four workgroups per CU.)
-    const bool lds_bvh = !MULTI && sizeof(R) == sizeof(float) && p.obj.bvh_leaves <= LDS_BVH_LEAVES;
+    // per-triangle bookkeeping.
+    const bool lds_bvh = !MULTI && p.obj.bvh_leaves <= LDS_BVH_LEAVES;
     unsigned char *bvh_lds = (unsigned char *)(s_code + STACK * blockDim.x);
     if (lds_bvh) {
         uint32_t *dst = (uint32_t *)bvh_lds;
@@ -1939,9 +1941,7 @@ template <class R> static size_t trace_lds_bytes(unsigned block)
 // The LDS copy of a small BVH in single-object launches (k_trace's lds_bvh).
 template <class R> static size_t bvh_lds_bytes(const ObjectDev &obj)
 {
-    return sizeof(R) == sizeof(float) && obj.bvh_leaves <= LDS_BVH_LEAVES
-               ? (sizeof(BvhNode) + sizeof(int)) * LDS_BVH_LEAVES
-               : 0;
+    return obj.bvh_leaves <= LDS_BVH_LEAVES ? (sizeof(BvhNode) + sizeof(int)) * LDS_BVH_LEAVES : 0;
 }
 
 // Persistent single-object launches: workgroups per CU when the runtime
