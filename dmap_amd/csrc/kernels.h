@@ -25,7 +25,8 @@ hipError_t launch_build_fold(const PyramidOut &P, const float *nodes, double *sc
 hipError_t launch_build_direct(const PyramidOut &P, const float *nodes, hipStream_t stream);
 
 // triangle_height_range (node_bounds.cpp:218-249) for every triangle: one
-// thread per triangle walks the min-max channel over its footprint.
+// workgroup per triangle walks the min-max channel over its footprint, the
+// walk split into cells a few levels below the root cells.
 hipError_t launch_height_range(const PyramidView &P, const TriRecord *tris, int n_tri, float *h_range,
                                hipStream_t stream);
 

@@ -1883,51 +1883,110 @@ __device__ inline int classify_d(const TriRecord &T, double cx, double cy, doubl
 }
 
 // height_range_walk / triangle_height_range (node_bounds.cpp:218-249).
-__global__ void k_height_range(PyramidView P, const TriRecord *tris, int n_tri, float *h_range)
+// One workgroup per triangle. The reference's walk (height_range_walk,
+// node_bounds.cpp:216-233: a cell inside the domain contributes its own
+// min-max node, a straddling cell descends, down to the leaf level) is split
+// at level L0 = root_level - D: each task is one cell of a root cell's
+// subtree at L0. A task first classifies the cell's ancestors from the root
+// down; an outside ancestor ends it, an inside one contributes its node and
+// ends it (the serial walk stops there too), and a cell whose ancestors all
+// straddle is walked depth first as the serial walk would. The tasks
+// together visit exactly the serial walk's nodes, some of them more than
+// once, which min and max do not mind.
+constexpr int HR_BLOCK = 256;
+constexpr int HR_SPLIT = 5; // levels below the root cells handed out as tasks (4^5 per root)
+
+__global__ void __launch_bounds__(HR_BLOCK) k_height_range(PyramidView P, const TriRecord *tris, int n_tri,
+                                                          float *h_range)
 {
-    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int k = blockIdx.x;
     if (k >= n_tri)
         return;
     const TriRecord &T = tris[k];
-    int stack_l[64], stack_i[64], stack_j[64];
-    int sp = 0;
-    for (int r = T.n_roots - 1; r >= 0; --r) {
-        stack_l[sp] = T.root_level;
-        stack_i[sp] = T.root_i[r];
-        stack_j[sp] = T.root_j[r];
-        ++sp;
-    }
+    const int L = T.root_level;
+    const int D = L < HR_SPLIT ? L : HR_SPLIT;
+    const int L0 = L - D;
+    const int per_root = 1 << (2 * D);
+    const int tasks = T.n_roots * per_root;
     float lo = INFINITY, hi = -INFINITY;
-    while (sp > 0) {
-        --sp;
-        int level = stack_l[sp], i = stack_i[sp], j = stack_j[sp];
-        double w = ldexp(1.0, level) / P.n_leaf;
-        int ov = classify_d(T, (i + 0.5) * w, (j + 0.5) * w, 0.5 * w);
-        if (ov == 0)
-            continue;
-        if (ov == 1 || level == 0) {
-            float2 mm = fetch_mm(P, level, i, j);
-            lo = fminf(lo, mm.x);
-            hi = fmaxf(hi, mm.y);
-            continue;
+    int stack_l[64], stack_i[64], stack_j[64];
+    for (int t = threadIdx.x; t < tasks; t += HR_BLOCK) {
+        const int r = t / per_root, m = t % per_root;
+        int di = 0, dj = 0;
+        for (int b = 0; b < D; ++b) {
+            di |= ((m >> (2 * b)) & 1) << b;
+            dj |= ((m >> (2 * b + 1)) & 1) << b;
         }
-        if (sp + 4 > 64)
-            continue; // cannot happen for levels below 20
-        for (int ch = 3; ch >= 0; --ch) {
-            stack_l[sp] = level - 1;
-            stack_i[sp] = 2 * i + (ch & 1);
-            stack_j[sp] = 2 * j + (ch >> 1);
-            ++sp;
+        const int ci = T.root_i[r] * (1 << D) + di, cj = T.root_j[r] * (1 << D) + dj;
+        bool done = false;
+        for (int lv = L; lv > L0 && !done; --lv) {
+            const int ai = ci >> (lv - L0), aj = cj >> (lv - L0);
+            const double w = ldexp(1.0, lv) / P.n_leaf;
+            const int ov = classify_d(T, (ai + 0.5) * w, (aj + 0.5) * w, 0.5 * w);
+            if (ov == 0) {
+                done = true;
+            } else if (ov == 1) {
+                const float2 mm = fetch_mm(P, lv, ai, aj);
+                lo = fminf(lo, mm.x);
+                hi = fmaxf(hi, mm.y);
+                done = true;
+            }
+        }
+        if (done)
+            continue;
+        int sp = 0;
+        stack_l[sp] = L0;
+        stack_i[sp] = ci;
+        stack_j[sp] = cj;
+        ++sp;
+        while (sp > 0) {
+            --sp;
+            const int level = stack_l[sp], i = stack_i[sp], j = stack_j[sp];
+            const double w = ldexp(1.0, level) / P.n_leaf;
+            const int ov = classify_d(T, (i + 0.5) * w, (j + 0.5) * w, 0.5 * w);
+            if (ov == 0)
+                continue;
+            if (ov == 1 || level == 0) {
+                const float2 mm = fetch_mm(P, level, i, j);
+                lo = fminf(lo, mm.x);
+                hi = fmaxf(hi, mm.y);
+                continue;
+            }
+            if (sp + 4 > 64)
+                continue; // cannot happen below level 20
+            for (int ch = 3; ch >= 0; --ch) {
+                stack_l[sp] = level - 1;
+                stack_i[sp] = 2 * i + (ch & 1);
+                stack_j[sp] = 2 * j + (ch >> 1);
+                ++sp;
+            }
         }
     }
-    h_range[2 * k] = lo;
-    h_range[2 * k + 1] = hi;
+    // The workgroup's min and max.
+    for (int o = 32; o > 0; o >>= 1) {
+        lo = fminf(lo, __shfl_xor(lo, o));
+        hi = fmaxf(hi, __shfl_xor(hi, o));
+    }
+    __shared__ float s_lo[HR_BLOCK / 64], s_hi[HR_BLOCK / 64];
+    if ((threadIdx.x & 63) == 0) {
+        s_lo[threadIdx.x >> 6] = lo;
+        s_hi[threadIdx.x >> 6] = hi;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int q = 1; q < HR_BLOCK / 64; ++q) {
+            lo = fminf(lo, s_lo[q]);
+            hi = fmaxf(hi, s_hi[q]);
+        }
+        h_range[2 * k] = lo;
+        h_range[2 * k + 1] = hi;
+    }
 }
 
 hipError_t launch_height_range(const PyramidView &P, const TriRecord *tris, int n_tri, float *h_range,
                                hipStream_t stream)
 {
-    hipLaunchKernelGGL(k_height_range, dim3((n_tri + 63) / 64), dim3(64), 0, stream, P, tris, n_tri, h_range);
+    hipLaunchKernelGGL(k_height_range, dim3(n_tri), dim3(HR_BLOCK), 0, stream, P, tris, n_tri, h_range);
     return hipGetLastError();
 }
 
