@@ -50,7 +50,7 @@ __device__ void store_node(const PyramidOut &P, int level, int i, int j, const D
     double hmin = off != 0.0 ? nd.hmin + off : nd.hmin;
     double hmax = off != 0.0 ? nd.hmax + off : nd.hmax;
     float h0f = (float)h0, guf = (float)nd.gu, gvf = (float)nd.gv;
-    double half = 0.5 * ldexp(1.0, level) / P.n_leaf;
+    double half = ldexp(0.5, level - (__ffs(P.n_leaf) - 1)); // 0.5 * 2^level / n_leaf, exactly
     double egu = fabs(nd.gu - (double)guf), egv = fabs(nd.gv - (double)gvf);
     double r = nd.r + fabs(h0 - (double)h0f) + (egu + egv) * half;
     double ru = nd.ru + egu, rv = nd.rv + egv;
@@ -64,14 +64,17 @@ __device__ void store_node(const PyramidOut &P, int level, int i, int j, const D
 }
 
 // leaf_level (taylor_pyramid.cpp:45-67) for one texel from its four scaled nodes.
+// n is a power of two, so w = 1 / n is exact and dividing by w or w^2 is
+// the exact scaling by n or n^2: the products below equal the reference's
+// quotients bit for bit.
 __device__ inline DNode leaf_node(double v00, double v10, double v01, double v11, int n)
 {
-    double w = 1.0 / n, s = 0.5 * w;
+    const double dn = (double)n, w = ldexp(1.0, -(__ffs(n) - 1)), s = 0.5 * w;
     DNode o;
     o.h0 = 0.25 * (v00 + v10 + v01 + v11);
-    o.gu = 0.5 * ((v10 - v00) + (v11 - v01)) / w;
-    o.gv = 0.5 * ((v01 - v00) + (v11 - v10)) / w;
-    double c3 = (v00 - v10 - v01 + v11) / (w * w);
+    o.gu = 0.5 * ((v10 - v00) + (v11 - v01)) * dn;
+    o.gv = 0.5 * ((v01 - v00) + (v11 - v10)) * dn;
+    double c3 = (v00 - v10 - v01 + v11) * (dn * dn);
     o.r = fabs(c3) * s * s;
     o.ru = fabs(c3) * s;
     o.rv = fabs(c3) * s;
@@ -84,7 +87,7 @@ __device__ inline DNode leaf_node(double v00, double v10, double v01, double v11
 // (a along v, b along u) of a parent on a level of m cells per side.
 __device__ DNode fold_node(const DNode c[4], int m)
 {
-    double s = 0.5 / m;
+    double s = ldexp(0.5, -(__ffs(m) - 1)); // 0.5 / m, m a power of two
     const double off[2] = {-0.5 * s, 0.5 * s};
     double gu_hi = -INFINITY, gu_lo = INFINITY, gv_hi = -INFINITY, gv_lo = INFINITY;
     double h_min = INFINITY, h_max = -INFINITY;
@@ -270,17 +273,18 @@ __global__ void __launch_bounds__(256) k_fold_pass(PyramidOut P, const double *_
 
 // ---------------------------------------------------------------- direct
 
-// Texel coefficients c1, c2, c3 (taylor_pyramid.cpp:20-39) of texel (i, j).
+// Texel coefficients c1, c2, c3 (taylor_pyramid.cpp:20-39) of texel (i, j);
+// the divisions by w = 1 / n are the exact scalings by n and n^2.
 __device__ inline void texel_c(const float *nodes, int W, double scale, int n, int i, int j, double &c1, double &c2,
                                double &c3)
 {
-    double w = 1.0 / n;
+    const double dn = (double)n;
     double v00 = (double)nodes[(size_t)j * W + i] * scale, v10 = (double)nodes[(size_t)j * W + i + 1] * scale;
     double v01 = (double)nodes[(size_t)(j + 1) * W + i] * scale;
     double v11 = (double)nodes[(size_t)(j + 1) * W + i + 1] * scale;
-    c1 = 0.5 * ((v10 - v00) + (v11 - v01)) / w;
-    c2 = 0.5 * ((v01 - v00) + (v11 - v10)) / w;
-    c3 = (v00 - v10 - v01 + v11) / (w * w);
+    c1 = 0.5 * ((v10 - v00) + (v11 - v01)) * dn;
+    c2 = 0.5 * ((v01 - v00) + (v11 - v10)) * dn;
+    c3 = (v00 - v10 - v01 + v11) * (dn * dn);
 }
 
 // direct_level (taylor_pyramid.cpp:151-201), one thread per cell (fine levels).
