@@ -38,6 +38,8 @@ constexpr double LIGHT_EPS = 1e-4; // shadow rays stop this short of the light
 constexpr double SPAWN_EPS = 1e-4; // new rays start this far off the surface
 constexpr double PI = 3.141592653589793;
 constexpr int BLOCK = 256;
+// The shading kernel's workgroups: 16 waves share one queue atomic.
+constexpr int SHADE_BLOCK = 1024;
 // Paths per wavefront chunk: about 270 B of queues each.
 constexpr int64_t MAX_CHUNK = (int64_t)1 << 22;
 
@@ -89,7 +91,7 @@ __device__ __forceinline__ double uniform01(uint32_t pid, uint32_t bounce, uint3
 __device__ __forceinline__ void block_claim(bool want0, bool want1, uint32_t *counters, uint32_t &slot0,
                                             uint32_t &slot1)
 {
-    __shared__ uint32_t s_cnt[2][BLOCK / 64];
+    __shared__ uint32_t s_cnt[2][SHADE_BLOCK / 64];
     __shared__ uint32_t s_base[2];
     const uint64_t m0 = __ballot(want0), m1 = __ballot(want1);
     const int lane = __lane_id(), wv = threadIdx.x >> 6;
@@ -101,7 +103,7 @@ __device__ __forceinline__ void block_claim(bool want0, bool want1, uint32_t *co
     if (threadIdx.x == 0) {
         for (int q = 0; q < 2; ++q) {
             uint32_t tot = 0;
-            for (int w = 0; w < BLOCK / 64; ++w)
+            for (int w = 0; w < SHADE_BLOCK / 64; ++w)
                 tot += s_cnt[q][w];
             s_base[q] = tot ? atomicAdd(counters + q, tot) : 0u;
         }
@@ -182,9 +184,9 @@ struct ShadeArgs
     uint32_t seed;
 };
 
-__global__ void k_pt_shade(ShadeArgs a)
+__global__ void __launch_bounds__(SHADE_BLOCK) k_pt_shade(ShadeArgs a)
 {
-    int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+    int64_t i = (int64_t)blockIdx.x * SHADE_BLOCK + threadIdx.x;
     bool live = i < a.n;
     bool cont = false, nee = false;
     uint32_t pid = 0;
@@ -474,7 +476,8 @@ int dmap_pathtrace(const dmap_object *obj, const double *cam_o, const double *ca
             a.last = last ? 1 : 0;
             a.mis = strategy == 0 ? 1 : 0;
             a.seed = seed;
-            hipLaunchKernelGGL(k_pt_shade, dim3(grid_of(live)), dim3(BLOCK), 0, s, a);
+            hipLaunchKernelGGL(k_pt_shade, dim3((unsigned)((live + SHADE_BLOCK - 1) / SHADE_BLOCK)), dim3(SHADE_BLOCK),
+                               0, s, a);
             if (last)
                 break;
             if (hipMemcpyAsync(pinned, W.counters, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, s) != hipSuccess ||
