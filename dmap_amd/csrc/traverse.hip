@@ -58,6 +58,19 @@ constexpr int BLOCK = 256;
 
 template <class R> __device__ inline R pad_of(R magnitude) { return (R)KPAD * Num<R>::eps * magnitude; }
 template <class R> __device__ inline R tolerance(R t_scale) { return Num<R>::tol_rel * fmax((R)1, t_scale); }
+// a / b for a bound that pad_of widens afterwards. In fp32 it is a times the
+// hardware reciprocal (v_rcp_f32, within 1 ulp for a normal result): with
+// the rounding of the numerator and of the product the result lies within
+// 4 unit roundoffs of the exact quotient, inside the 8 of pad_of. A |b| of
+// 2^126 or more (a denormal reciprocal) is scaled by 2^-64 first, exactly;
+// a product that overflows where the quotient does not only widens the
+// interval. In fp64 it is the division.
+template <class R> __device__ inline R padded_div(R a, R b) { return a / b; }
+template <> __device__ inline float padded_div<float>(float a, float b)
+{
+    const float k = fabsf(b) >= 0x1p126f ? 0x1p-64f : 1.0f;
+    return (a * __builtin_amdgcn_rcpf(b * k)) * k;
+}
 
 // Aabb3::clip (node_bounds.cpp:25-44) on one axis, widened by rounding room.
 template <class R> __device__ inline bool clip_axis(R o, R d, R inv, R lo, R hi, R &t0, R &t1)
@@ -103,11 +116,6 @@ template <class R> struct IvR
     R lo, hi;
     __device__ R mag() const { return fmax(fabs(lo), fabs(hi)); }
 };
-template <class R> __device__ inline IvR<R> smul(R s, IvR<R> a)
-{
-    return s >= (R)0 ? IvR<R>{s * a.lo, s * a.hi} : IvR<R>{s * a.hi, s * a.lo};
-}
-template <class R> __device__ inline IvR<R> iadd(IvR<R> a, IvR<R> b) { return {a.lo + b.lo, a.hi + b.hi}; }
 
 // The normalized normal's affine form in tangent space over one cell:
 // centre mc, symbol coefficients ma, mb, remainder mk per axis.
@@ -359,20 +367,23 @@ __device__ bool cell_test(const TraceParams &p, const PyramidView &P, const TriR
     bool want_box = p.bound == 0 || p.bound == 2 || !read_slab;
     bool want_slab = p.bound != 0 && read_slab;
 
-    R b0 = t_min, b1 = t_max;
+    // The box's x and y clips are the column's (slab_interval,
+    // node_bounds.cpp:137-179): both start from [t_min, t_max], so the two
+    // are clipped once and shared.
+    R t0 = t_min, t1 = t_max;
+    if (!clip_axis<R>(wk.o[0], wk.d[0], wk.inv[0], blo[0], bhi[0], t0, t1) ||
+        !clip_axis<R>(wk.o[1], wk.d[1], wk.inv[1], blo[1], bhi[1], t0, t1))
+        return false;
+    R b0 = t0, b1 = t1;
     if (want_box) {
-        if (!clip_axis<R>(wk.o[0], wk.d[0], wk.inv[0], blo[0], bhi[0], b0, b1) ||
-            !clip_axis<R>(wk.o[1], wk.d[1], wk.inv[1], blo[1], bhi[1], b0, b1) ||
-            !clip_axis<R>(wk.o[2], wk.d[2], wk.inv[2], blo[2], bhi[2], b0, b1) || !close_interval<R>(b0, b1))
+        if (!clip_axis<R>(wk.o[2], wk.d[2], wk.inv[2], blo[2], bhi[2], b0, b1) || !close_interval<R>(b0, b1))
             return false;
         if (!want_slab)
             return true;
     }
 
-    // slab_interval (node_bounds.cpp:137-179): the column first.
-    R t0 = t_min, t1 = t_max;
-    if (!clip_axis<R>(wk.o[0], wk.d[0], wk.inv[0], blo[0], bhi[0], t0, t1) ||
-        !clip_axis<R>(wk.o[1], wk.d[1], wk.inv[1], blo[1], bhi[1], t0, t1) || !close_interval<R>(t0, t1))
+    // slab_interval: the column.
+    if (!close_interval<R>(t0, t1))
         return false;
     R anchor[3];
     for (int q = 0; q < 3; ++q)
@@ -384,16 +395,24 @@ __device__ bool cell_test(const TraceParams &p, const PyramidView &P, const TriR
         gv = 0;
     }
     R ucx = ((R)ci + (R)0.5) * w, ucy = ((R)cj + (R)0.5) * w; // nb.centre
-    IvR<R> mx{m.c[0] - (fabs(m.a[0]) + fabs(m.b[0]) + m.k[0]), m.c[0] + (fabs(m.a[0]) + fabs(m.b[0]) + m.k[0])};
-    IvR<R> my{m.c[1] - (fabs(m.a[1]) + fabs(m.b[1]) + m.k[1]), m.c[1] + (fabs(m.a[1]) + fabs(m.b[1]) + m.k[1])};
-    IvR<R> mz{m.c[2] - (fabs(m.a[2]) + fabs(m.b[2]) + m.k[2]), m.c[2] + (fabs(m.a[2]) + fabs(m.b[2]) + m.k[2])};
-    IvR<R> A = iadd(iadd(IvR<R>{(R)1, (R)1}, smul(gu, mx)), smul(gv, my));
-    IvR<R> B = smul(-gu, mz), C = smul(-gv, mz);
+    // The interval products and sums in centre-radius form: s [c - r, c + r]
+    // = [s c - |s| r, s c + |s| r] and sums add centres and radii, so in
+    // exact arithmetic the intervals equal the endpoint forms, with no sign
+    // selects; the rounding of either form is inside pad_of(err) below.
+    const R xc = m.c[0], yc = m.c[1], zc = m.c[2];
+    const R xr = fabs(m.a[0]) + fabs(m.b[0]) + m.k[0], yr = fabs(m.a[1]) + fabs(m.b[1]) + m.k[1],
+            zr = fabs(m.a[2]) + fabs(m.b[2]) + m.k[2];
+    const IvR<R> mx{xc - xr, xc + xr}, my{yc - yr, yc + yr}, mz{zc - zr, zc + zr};
+    const R Ac = (R)1 + gu * xc + gv * yc, Ar = fabs(gu) * xr + fabs(gv) * yr;
+    const R Bc = -gu * zc, Br = fabs(gu) * zr, Cc = -gv * zc, Cr = fabs(gv) * zr;
     R dconst = -(h0 - gu * ucx - gv * ucy);
-    IvR<R> D = smul(dconst, mz);
+    const R Dc = dconst * zc, Dr = fabs(dconst) * zr;
     R Rr = rr * mz.mag();
-    IvR<R> F0 = iadd(iadd(iadd(smul(anchor[2], A), smul(anchor[0], B)), smul(anchor[1], C)), D);
-    IvR<R> F1 = iadd(iadd(smul(wk.d[2], A), smul(wk.d[0], B)), smul(wk.d[1], C));
+    const R F0c = anchor[2] * Ac + anchor[0] * Bc + anchor[1] * Cc + Dc;
+    const R F0r = fabs(anchor[2]) * Ar + fabs(anchor[0]) * Br + fabs(anchor[1]) * Cr + Dr;
+    const R F1c = wk.d[2] * Ac + wk.d[0] * Bc + wk.d[1] * Cc;
+    const R F1r = fabs(wk.d[2]) * Ar + fabs(wk.d[0]) * Br + fabs(wk.d[1]) * Cr;
+    const IvR<R> F0{F0c - F0r, F0c + F0r}, F1{F1c - F1r, F1c + F1r};
     // Rounding room of everything above, folded into the remainder R.
     R magA = (R)1 + fabs(gu) * mx.mag() + fabs(gv) * my.mag(), magB = fabs(gu) * mz.mag(),
       magC = fabs(gv) * mz.mag();
@@ -404,15 +423,15 @@ __device__ bool cell_test(const TraceParams &p, const PyramidView &P, const TriR
     Rr += pad_of<R>(err);
     R s0 = 0, s1 = span;
     if (F1.lo > (R)0)
-        s1 = fmin(s1, (Rr - F0.lo) / F1.lo);
+        s1 = fmin(s1, padded_div<R>(Rr - F0.lo, F1.lo));
     else if (F1.lo < (R)0)
-        s0 = fmax(s0, (Rr - F0.lo) / F1.lo);
+        s0 = fmax(s0, padded_div<R>(Rr - F0.lo, F1.lo));
     else if (F0.lo > Rr)
         return false;
     if (F1.hi > (R)0)
-        s0 = fmax(s0, (-Rr - F0.hi) / F1.hi);
+        s0 = fmax(s0, padded_div<R>(-Rr - F0.hi, F1.hi));
     else if (F1.hi < (R)0)
-        s1 = fmin(s1, (-Rr - F0.hi) / F1.hi);
+        s1 = fmin(s1, padded_div<R>(-Rr - F0.hi, F1.hi));
     else if (F0.hi < -Rr)
         return false;
     s0 -= pad_of<R>(fabs(s0));
