@@ -352,10 +352,12 @@ __device__ bool cell_test(const TraceParams &p, const PyramidView &P, const TriR
     R blo[3], bhi[3];
     const R pc[3] = {cx, cy, (R)0}, pa[3] = {hx, (R)0, (R)0}, pb[3] = {(R)0, hy, (R)0};
     for (int q = 0; q < 3; ++q) {
-        R dev_m = fabs(m.a[q]) + fabs(m.b[q]) + m.k[q];
+        // Under a constant normal (CN) the symbol coefficients are zero and
+        // these are the same values without the zero terms.
+        R dev_m = CN ? m.k[q] : fabs(m.a[q]) + fabs(m.b[q]) + m.k[q];
         R c = pc[q] + hc * m.c[q];
-        R a = pa[q] + hc * m.a[q];
-        R b = pb[q] + hc * m.b[q];
+        R a = CN ? pa[q] : pa[q] + hc * m.a[q];
+        R b = CN ? pb[q] : pb[q] + hc * m.b[q];
         R k = fabs(m.c[q]) * hr + fabs(hc) * m.k[q] + hr * dev_m;
         R e = fabs(a) + fabs(b) + k;
         e += pad_of<R>(fabs(c) + e);
@@ -400,8 +402,9 @@ __device__ bool cell_test(const TraceParams &p, const PyramidView &P, const TriR
     // exact arithmetic the intervals equal the endpoint forms, with no sign
     // selects; the rounding of either form is inside pad_of(err) below.
     const R xc = m.c[0], yc = m.c[1], zc = m.c[2];
-    const R xr = fabs(m.a[0]) + fabs(m.b[0]) + m.k[0], yr = fabs(m.a[1]) + fabs(m.b[1]) + m.k[1],
-            zr = fabs(m.a[2]) + fabs(m.b[2]) + m.k[2];
+    const R xr = CN ? m.k[0] : fabs(m.a[0]) + fabs(m.b[0]) + m.k[0],
+            yr = CN ? m.k[1] : fabs(m.a[1]) + fabs(m.b[1]) + m.k[1],
+            zr = CN ? m.k[2] : fabs(m.a[2]) + fabs(m.b[2]) + m.k[2];
     const IvR<R> mx{xc - xr, xc + xr}, my{yc - yr, yc + yr}, mz{zc - zr, zc + zr};
     const R Ac = (R)1 + gu * xc + gv * yc, Ar = fabs(gu) * xr + fabs(gv) * yr;
     const R Bc = -gu * zc, Br = fabs(gu) * zr, Cc = -gv * zc, Cr = fabs(gv) * zr;
