@@ -72,6 +72,27 @@ template <> __device__ inline float padded_div<float>(float a, float b)
     return (a * __builtin_amdgcn_rcpf(b * k)) * k;
 }
 
+// clip_axis for the slab [c - e, c + e] given by its centre and radius:
+// the ray meets it over (c - o) / d -+ e / |d|, with no ordering of the two
+// ends. Rounding room: pad (8 unit roundoffs of |o| + |c| + e) on the
+// radius covers the reciprocal and the two products, and 8 unit roundoffs
+// of |tm| + tr on the parameters cover the final sums. An overflow gives
+// an infinite or NaN end, which fmax / fmin then ignore or keep, never
+// narrowing the interval below the exact one.
+template <class R> __device__ inline bool clip_centre(R o, R d, R inv, R c, R e, R &t0, R &t1)
+{
+    R pad = pad_of<R>(fabs(o) + fabs(c) + e);
+    if (d != (R)0) {
+        R tm = (c - o) * inv, tr = (e + pad) * fabs(inv);
+        tr += pad_of<R>(fabs(tm) + tr);
+        t0 = fmax(t0, tm - tr);
+        t1 = fmin(t1, tm + tr);
+    } else if (fabs(o - c) > e + pad) {
+        return false;
+    }
+    return true;
+}
+
 // Aabb3::clip (node_bounds.cpp:25-44) on one axis, widened by rounding room.
 template <class R> __device__ inline bool clip_axis(R o, R d, R inv, R lo, R hi, R &t0, R &t1)
 {
@@ -349,7 +370,7 @@ __device__ bool cell_test(const TraceParams &p, const PyramidView &P, const TriR
 
     R hc = (R)0.5 * ((R)mm.x + (R)mm.y), hr = (R)0.5 * ((R)mm.y - (R)mm.x);
     // box = hull(p + h m): p = (cx +- hx, cy +- hy, 0), h = hc +- hr.
-    R blo[3], bhi[3];
+    R bc[3], be[3]; // the box: centre +- radius per axis
     const R pc[3] = {cx, cy, (R)0}, pa[3] = {hx, (R)0, (R)0}, pb[3] = {(R)0, hy, (R)0};
     for (int q = 0; q < 3; ++q) {
         // Under a constant normal (CN) the symbol coefficients are zero and
@@ -361,8 +382,8 @@ __device__ bool cell_test(const TraceParams &p, const PyramidView &P, const TriR
         R k = fabs(m.c[q]) * hr + fabs(hc) * m.k[q] + hr * dev_m;
         R e = fabs(a) + fabs(b) + k;
         e += pad_of<R>(fabs(c) + e);
-        blo[q] = c - e;
-        bhi[q] = c + e;
+        bc[q] = c;
+        be[q] = e;
     }
 
     bool read_slab = level <= p.slab_max_level;
@@ -373,12 +394,12 @@ __device__ bool cell_test(const TraceParams &p, const PyramidView &P, const TriR
     // node_bounds.cpp:137-179): both start from [t_min, t_max], so the two
     // are clipped once and shared.
     R t0 = t_min, t1 = t_max;
-    if (!clip_axis<R>(wk.o[0], wk.d[0], wk.inv[0], blo[0], bhi[0], t0, t1) ||
-        !clip_axis<R>(wk.o[1], wk.d[1], wk.inv[1], blo[1], bhi[1], t0, t1))
+    if (!clip_centre<R>(wk.o[0], wk.d[0], wk.inv[0], bc[0], be[0], t0, t1) ||
+        !clip_centre<R>(wk.o[1], wk.d[1], wk.inv[1], bc[1], be[1], t0, t1))
         return false;
     R b0 = t0, b1 = t1;
     if (want_box) {
-        if (!clip_axis<R>(wk.o[2], wk.d[2], wk.inv[2], blo[2], bhi[2], b0, b1) || !close_interval<R>(b0, b1))
+        if (!clip_centre<R>(wk.o[2], wk.d[2], wk.inv[2], bc[2], be[2], b0, b1) || !close_interval<R>(b0, b1))
             return false;
         if (!want_slab)
             return true;
