@@ -57,7 +57,17 @@ constexpr int LDS_BVH_LEAVES = 128;
 constexpr int BLOCK = 256;
 
 template <class R> __device__ inline R pad_of(R magnitude) { return (R)KPAD * Num<R>::eps * magnitude; }
-template <class R> __device__ inline R tolerance(R t_scale) { return Num<R>::tol_rel * fmax((R)1, t_scale); }
+// max / min for the node test's bounds. In fp32 they propagate NaN
+// (v_maximum3_f32 / v_minimum3_f32), which needs no quieting of the
+// operands as fmax / fmin do. A NaN there comes only from an overflow, and
+// every comparison it reaches is false, so the test passes: the bound only
+// widens. In fp64 they are fmax / fmin.
+template <class R> __device__ inline R nmax(R a, R b) { return fmax(a, b); }
+template <class R> __device__ inline R nmin(R a, R b) { return fmin(a, b); }
+template <> __device__ inline float nmax<float>(float a, float b) { return __builtin_elementwise_maximum(a, b); }
+template <> __device__ inline float nmin<float>(float a, float b) { return __builtin_elementwise_minimum(a, b); }
+
+template <class R> __device__ inline R tolerance(R t_scale) { return Num<R>::tol_rel * nmax((R)1, t_scale); }
 // a / b for a bound that pad_of widens afterwards. In fp32 it is a times the
 // hardware reciprocal (v_rcp_f32, within 1 ulp for a normal result): with
 // the rounding of the numerator and of the product the result lies within
@@ -77,16 +87,16 @@ template <> __device__ inline float padded_div<float>(float a, float b)
 // ends. Rounding room: pad (8 unit roundoffs of |o| + |c| + e) on the
 // radius covers the reciprocal and the two products, and 8 unit roundoffs
 // of |tm| + tr on the parameters cover the final sums. An overflow gives
-// an infinite or NaN end, which fmax / fmin then ignore or keep, never
-// narrowing the interval below the exact one.
+// an infinite or NaN end: an infinite one is the exact end's rounding
+// outward, and a NaN one passes the test (nmax).
 template <class R> __device__ inline bool clip_centre(R o, R d, R inv, R c, R e, R &t0, R &t1)
 {
     R pad = pad_of<R>(fabs(o) + fabs(c) + e);
     if (d != (R)0) {
         R tm = (c - o) * inv, tr = (e + pad) * fabs(inv);
         tr += pad_of<R>(fabs(tm) + tr);
-        t0 = fmax(t0, tm - tr);
-        t1 = fmin(t1, tm + tr);
+        t0 = nmax(t0, tm - tr);
+        t1 = nmin(t1, tm + tr);
     } else if (fabs(o - c) > e + pad) {
         return false;
     }
@@ -96,7 +106,7 @@ template <class R> __device__ inline bool clip_centre(R o, R d, R inv, R c, R e,
 // Aabb3::clip (node_bounds.cpp:25-44) on one axis, widened by rounding room.
 template <class R> __device__ inline bool clip_axis(R o, R d, R inv, R lo, R hi, R &t0, R &t1)
 {
-    R pad = pad_of<R>(fabs(o) + fmax(fabs(lo), fabs(hi)));
+    R pad = pad_of<R>(fabs(o) + nmax(fabs(lo), fabs(hi)));
     lo -= pad;
     hi += pad;
     if (d != (R)0) {
@@ -108,8 +118,8 @@ template <class R> __device__ inline bool clip_axis(R o, R d, R inv, R lo, R hi,
         }
         ta -= pad_of<R>(fabs(ta));
         tb += pad_of<R>(fabs(tb));
-        t0 = fmax(t0, ta);
-        t1 = fmin(t1, tb);
+        t0 = nmax(t0, ta);
+        t1 = nmin(t1, tb);
     } else if (o < lo || o > hi) {
         return false;
     }
@@ -118,7 +128,7 @@ template <class R> __device__ inline bool clip_axis(R o, R d, R inv, R lo, R hi,
 
 template <class R> __device__ inline bool close_interval(R &t0, R &t1)
 {
-    if (t0 > t1 + tolerance<R>(fmax(fabs(t0), fabs(t1))))
+    if (t0 > t1 + tolerance<R>(nmax(fabs(t0), fabs(t1))))
         return false;
     if (t0 > t1) {
         R x = t0;
@@ -135,7 +145,7 @@ struct Iv
 template <class R> struct IvR
 {
     R lo, hi;
-    __device__ R mag() const { return fmax(fabs(lo), fabs(hi)); }
+    __device__ R mag() const { return nmax(fabs(lo), fabs(hi)); }
 };
 
 // The normalized normal's affine form in tangent space over one cell:
@@ -345,8 +355,8 @@ __device__ bool cell_test(const TraceParams &p, const PyramidView &P, const TriR
     R w = ldexp((R)1, level - (__ffs(P.n_leaf) - 1)); // 2^level / n_leaf, n_leaf a power of two
     R clo_x = (R)ci * w, clo_y = (R)cj * w;
     // The cell clipped to the triangle's texture bounding box (node_bounds.cpp:94-99).
-    R rlo_x = fmax(clo_x, (R)T.dlo[0]), rlo_y = fmax(clo_y, (R)T.dlo[1]);
-    R rhi_x = fmin(clo_x + w, (R)T.dhi[0]), rhi_y = fmin(clo_y + w, (R)T.dhi[1]);
+    R rlo_x = nmax(clo_x, (R)T.dlo[0]), rlo_y = nmax(clo_y, (R)T.dlo[1]);
+    R rhi_x = nmin(clo_x + w, (R)T.dhi[0]), rhi_y = nmin(clo_y + w, (R)T.dhi[1]);
     if (sizeof(R) == 8) {
         rlo_x = fmax(clo_x, (R)T.dlo_d[0]);
         rlo_y = fmax(clo_y, (R)T.dlo_d[1]);
@@ -354,7 +364,7 @@ __device__ bool cell_test(const TraceParams &p, const PyramidView &P, const TriR
         rhi_y = fmin(clo_y + w, (R)T.dhi_d[1]);
     }
     R cx = (R)0.5 * (rlo_x + rhi_x), cy = (R)0.5 * (rlo_y + rhi_y);
-    R hx = fmax((R)0.5 * (rhi_x - rlo_x), (R)0), hy = fmax((R)0.5 * (rhi_y - rlo_y), (R)0);
+    R hx = nmax((R)0.5 * (rhi_x - rlo_x), (R)0), hy = nmax((R)0.5 * (rhi_y - rlo_y), (R)0);
 
     MForm<R> m;
     if (CN || T.const_normal) {
@@ -447,15 +457,15 @@ __device__ bool cell_test(const TraceParams &p, const PyramidView &P, const TriR
     Rr += pad_of<R>(err);
     R s0 = 0, s1 = span;
     if (F1.lo > (R)0)
-        s1 = fmin(s1, padded_div<R>(Rr - F0.lo, F1.lo));
+        s1 = nmin(s1, padded_div<R>(Rr - F0.lo, F1.lo));
     else if (F1.lo < (R)0)
-        s0 = fmax(s0, padded_div<R>(Rr - F0.lo, F1.lo));
+        s0 = nmax(s0, padded_div<R>(Rr - F0.lo, F1.lo));
     else if (F0.lo > Rr)
         return false;
     if (F1.hi > (R)0)
-        s0 = fmax(s0, padded_div<R>(-Rr - F0.hi, F1.hi));
+        s0 = nmax(s0, padded_div<R>(-Rr - F0.hi, F1.hi));
     else if (F1.hi < (R)0)
-        s1 = fmin(s1, padded_div<R>(-Rr - F0.hi, F1.hi));
+        s1 = nmin(s1, padded_div<R>(-Rr - F0.hi, F1.hi));
     else if (F0.hi < -Rr)
         return false;
     s0 -= pad_of<R>(fabs(s0));
@@ -471,7 +481,7 @@ __device__ bool cell_test(const TraceParams &p, const PyramidView &P, const TriR
     if (!want_box)
         return true;
     // both_interval (node_bounds.cpp:181-197).
-    R u0 = fmax(b0, st0), u1 = fmin(b1, st1);
+    R u0 = nmax(b0, st0), u1 = nmin(b1, st1);
     return close_interval<R>(u0, u1);
 }
 
