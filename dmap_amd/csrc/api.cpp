@@ -337,8 +337,15 @@ bool is_pow2(int n) { return n >= 1 && (n & (n - 1)) == 0; }
 // to child 2i, so a subtree of capacity C (a power of two) never receives
 // more than C triangles. It stands in for the reference's embree scene over
 // the same boxes (ObjectIntersector::build_bvh, displaced_intersector.cpp:540-554).
-// Writes the subtree's box (empty: lo > hi) to blo, bhi.
-void build_bvh_node(std::vector<BvhNode> &nodes, std::vector<int> &leaf, const std::vector<float> &boxes,
+// Writes the subtree's box (empty: lo > hi) to blo, bhi. Nodes are built
+// with their children's boxes as lo / hi and stored as centre / radius
+// (bvh_centre_radius).
+struct BvhBuildNode
+{
+    float lo[2][3], hi[2][3];
+    int axis, empty;
+};
+void build_bvh_node(std::vector<BvhBuildNode> &nodes, std::vector<int> &leaf, const std::vector<float> &boxes,
                     std::vector<int> &idx, uint32_t node, int lo, int hi, uint32_t P, float blo[3], float bhi[3])
 {
     for (int a = 0; a < 3; ++a) {
@@ -373,7 +380,7 @@ void build_bvh_node(std::vector<BvhNode> &nodes, std::vector<int> &leaf, const s
         std::nth_element(idx.begin() + lo, idx.begin() + mid, idx.begin() + hi, [&](int x, int y) {
             return boxes[6 * x + axis] + boxes[6 * x + 3 + axis] < boxes[6 * y + axis] + boxes[6 * y + 3 + axis];
         });
-    BvhNode nd{};
+    BvhBuildNode nd{};
     nd.axis = axis;
     nd.empty = (mid > lo ? 0 : 1) | (hi > mid ? 0 : 2);
     build_bvh_node(nodes, leaf, boxes, idx, 2 * node, lo, mid, P, nd.lo[0], nd.hi[0]);
@@ -389,6 +396,22 @@ void build_bvh_node(std::vector<BvhNode> &nodes, std::vector<int> &leaf, const s
             bhi[a] = std::max(bhi[a], nd.hi[c][a]);
         }
     nodes[node] = nd;
+}
+
+// The device form of a box [lo, hi]: fp32 centre c and radius r per axis
+// with [c - r, c + r] holding [lo, hi]. The centre is rounded to fp32, the
+// radius (formed in double) rounded up and then raised by one more ulp,
+// which covers the rounding of the double difference.
+void bvh_centre_radius(const float lo[3], const float hi[3], float c[3], float r[3])
+{
+    for (int a = 0; a < 3; ++a) {
+        c[a] = (float)(0.5 * ((double)lo[a] + (double)hi[a]));
+        const double rd = std::max((double)hi[a] - (double)c[a], (double)c[a] - (double)lo[a]);
+        float rf = (float)rd;
+        if ((double)rf < rd)
+            rf = std::nextafter(rf, INFINITY);
+        r[a] = std::nextafter(rf, INFINITY);
+    }
 }
 
 // Leaves of the heap-ordered BVH over n triangles: the power of two >= n.
@@ -704,12 +727,21 @@ static int object_bounds(dmap_object *o, const std::vector<HostTri> &tris)
     for (int k = 0; k < n_tri; ++k)
         idx[k] = k;
     const int P = o->bvh_leaves;
-    o->bvh_host.assign((size_t)P, BvhNode{});
+    std::vector<BvhBuildNode> build((size_t)P, BvhBuildNode{});
     o->bvh_leaf_host.assign((size_t)P, -1);
-    BvhNode &root = o->bvh_host[0];
-    build_bvh_node(o->bvh_host, o->bvh_leaf_host, o->boxes_host, idx, 1, 0, n_tri, (uint32_t)P, root.lo[0],
-                   root.hi[0]);
+    BvhBuildNode &root = build[0];
+    build_bvh_node(build, o->bvh_leaf_host, o->boxes_host, idx, 1, 0, n_tri, (uint32_t)P, root.lo[0], root.hi[0]);
     root.empty = 2;
+    o->bvh_host.assign((size_t)P, BvhNode{});
+    for (int i = 0; i < P; ++i) {
+        const BvhBuildNode &b = build[(size_t)i];
+        BvhNode &nd = o->bvh_host[(size_t)i];
+        nd.axis = b.axis;
+        nd.empty = b.empty;
+        for (int c = 0; c < 2; ++c)
+            if (!(b.empty & (1 << c)))
+                bvh_centre_radius(b.lo[c], b.hi[c], nd.ctr[c], nd.rad[c]);
+    }
     o->const_normal = 1;
     for (int k = 0; k < n_tri; ++k)
         o->const_normal &= o->host[k].const_normal;

@@ -50,13 +50,15 @@ struct StatsDev
 // children 2i and 2i + 1, and each of the leaves i in [P, 2P) holds one
 // triangle or none (bvh_leaf[i - P] = triangle or -1). An internal node
 // holds its two children's boxes, so one 64-byte read serves both tests;
-// node 0 holds the root's own box in lo[0], hi[0]. The traversal needs no
-// parent links or stack: the path to a node is its index.
+// node 0 holds the root's own box in ctr[0], rad[0]. The traversal needs no
+// parent links or stack: the path to a node is its index. A box is stored
+// as centre +- radius per axis, rounded so that it holds the box it stands
+// for, which is the form the ray test uses.
 struct BvhNode
 {
-    float lo[2][3], hi[2][3]; // the boxes of children 2i (lower centres along axis) and 2i + 1
-    int axis;                 // the split axis
-    int empty;                // bit c: child 2i + c holds no triangle
+    float ctr[2][3], rad[2][3]; // the boxes of children 2i (lower centres along axis) and 2i + 1
+    int axis;                   // the split axis
+    int empty;                  // bit c: child 2i + c holds no triangle
     int pad[2];
 };
 static_assert(sizeof(BvhNode) == 64, "one BVH node is 64 bytes");

@@ -68,10 +68,12 @@ template <> __device__ inline float nmax<float>(float a, float b) { return __bui
 template <> __device__ inline float nmin<float>(float a, float b) { return __builtin_elementwise_minimum(a, b); }
 
 template <class R> __device__ inline R tolerance(R t_scale) { return Num<R>::tol_rel * nmax((R)1, t_scale); }
-// a / b for a bound that pad_of widens afterwards. In fp32 it is a times the
-// hardware reciprocal (v_rcp_f32, within 1 ulp for a normal result): with
-// the rounding of the numerator and of the product the result lies within
-// 4 unit roundoffs of the exact quotient, inside the 8 of pad_of. A |b| of
+// a / b for a bound that pad_of widens afterwards (the slab test's
+// quotients, and the ray's inverse direction, which clip_axis and
+// clip_centre use). In fp32 it is a times the hardware reciprocal
+// (v_rcp_f32, within 1 ulp for a normal result): with the rounding of the
+// numerator and of the product the result lies within 4 unit roundoffs of
+// the exact quotient, inside the 8 of pad_of. A |b| of
 // 2^126 or more (a denormal reciprocal) is scaled by 2^-64 first, exactly;
 // a product that overflows where the quotient does not only widens the
 // interval. In fp64 it is the division.
@@ -988,7 +990,7 @@ __device__ bool leaf_two_triangle(const TraceParams &p, const TriRecord &T, cons
 
 // ------------------------------------------------------------------ kernel
 
-// Box of a BVH node or a triangle, clipped against [t_min, t_max].
+// A triangle's box (lo, hi), clipped against [t_min, t_max].
 template <class R>
 __device__ inline bool ray_box(const R o[3], const R d[3], const R inv[3], const float lo[3], const float hi[3], R t_min,
                                R t_max)
@@ -997,6 +999,17 @@ __device__ inline bool ray_box(const R o[3], const R d[3], const R inv[3], const
     return clip_axis<R>(o[0], d[0], inv[0], (R)lo[0], (R)hi[0], t0, t1) &&
            clip_axis<R>(o[1], d[1], inv[1], (R)lo[1], (R)hi[1], t0, t1) &&
            clip_axis<R>(o[2], d[2], inv[2], (R)lo[2], (R)hi[2], t0, t1) && close_interval<R>(t0, t1);
+}
+
+// A BVH box, stored as centre +- radius, clipped against [t_min, t_max].
+template <class R>
+__device__ inline bool ray_box_cr(const R o[3], const R d[3], const R inv[3], const float c[3], const float r[3],
+                                  R t_min, R t_max)
+{
+    R t0 = t_min, t1 = t_max;
+    return clip_centre<R>(o[0], d[0], inv[0], (R)c[0], (R)r[0], t0, t1) &&
+           clip_centre<R>(o[1], d[1], inv[1], (R)c[1], (R)r[1], t0, t1) &&
+           clip_centre<R>(o[2], d[2], inv[2], (R)c[2], (R)r[2], t0, t1) && close_interval<R>(t0, t1);
 }
 
 template <class R> __device__ inline R axis_of(const R v[3], int a) { return a == 0 ? v[0] : a == 1 ? v[1] : v[2]; }
@@ -1024,7 +1037,7 @@ __device__ __forceinline__ int bvh_next(const BvhNode *N, const int *leaf, uint3
         return -1;
     bool pop = true;
     if (cur == 0) {
-        if (!ray_box<R>(o, d, inv, N[0].lo[0], N[0].hi[0], t_min, t_max)) {
+        if (!ray_box_cr<R>(o, d, inv, N[0].ctr[0], N[0].rad[0], t_min, t_max)) {
             cur = BVH_DONE;
             return -1;
         }
@@ -1045,7 +1058,7 @@ __device__ __forceinline__ int bvh_next(const BvhNode *N, const int *leaf, uint3
             uint32_t far = (cur >> (dc - L - 1)) ^ 1u;
             cur = far;
             int c = (int)(far & 1u);
-            if (!ray_box<R>(o, d, inv, N[anc].lo[c], N[anc].hi[c], t_min, t_max))
+            if (!ray_box_cr<R>(o, d, inv, N[anc].ctr[c], N[anc].rad[c], t_min, t_max))
                 continue;
             pop = false;
         }
@@ -1057,8 +1070,8 @@ __device__ __forceinline__ int bvh_next(const BvhNode *N, const int *leaf, uint3
             continue;
         }
         const BvhNode &nd = N[cur];
-        bool h0 = !(nd.empty & 1) && ray_box<R>(o, d, inv, nd.lo[0], nd.hi[0], t_min, t_max);
-        bool h1 = !(nd.empty & 2) && ray_box<R>(o, d, inv, nd.lo[1], nd.hi[1], t_min, t_max);
+        bool h0 = !(nd.empty & 1) && ray_box_cr<R>(o, d, inv, nd.ctr[0], nd.rad[0], t_min, t_max);
+        bool h1 = !(nd.empty & 2) && ray_box_cr<R>(o, d, inv, nd.ctr[1], nd.rad[1], t_min, t_max);
         if (h0 && h1) {
             trail |= 1u << depth_of(cur);
             cur = 2 * cur + (axis_of(d, nd.axis) >= (R)0 ? 0u : 1u);
@@ -1333,7 +1346,7 @@ k_trace(TraceParams p)
                         t_min = (R)RY.tmin[rid];
                         t_best = (R)RY.tmax[rid];
                         for (int q = 0; q < 3; ++q)
-                            inv_f[q] = (R)1 / d_f[q];
+                            inv_f[q] = padded_div<R>((R)1, d_f[q]);
                         // A ray with a non-finite or zero component, or an empty
                         // (t_min, t_max), has no hit: the reference's walk finds none.
                         bool ok = isfinite(o_f[0]) && isfinite(o_f[1]) && isfinite(o_f[2]) && isfinite(d_f[0]) &&
@@ -1424,7 +1437,7 @@ k_trace(TraceParams p)
                 const double *row = T->T + 3 * q;
                 wk.o[q] = (R)(row[0] * rel[0] + row[1] * rel[1] + row[2] * rel[2]);
                 wk.d[q] = (R)(row[0] * (double)d_f[0] + row[1] * (double)d_f[1] + row[2] * (double)d_f[2]);
-                wk.inv[q] = (R)1 / wk.d[q];
+                wk.inv[q] = padded_div<R>((R)1, wk.d[q]);
                 omag = fmax(omag, fabs(wk.o[q]));
                 dmag = fmax(dmag, fabs(wk.d[q]));
             }
