@@ -715,9 +715,9 @@ __device__ bool sub_square_bound(const TriRecord &T, const Leaf<R> &L, R a, R b,
     if (!(F1c - e1 <= (R)0 && (R)0 <= F1c + e1) || !(F2c - e2 <= (R)0 && (R)0 <= F2c + e2))
         return false;
     R eg = fabs(Ga) + fabs(Gb) + Gk + pad_of<R>(fabs(L.cn) + local);
-    t_lo = (Gc - eg) / L.d_len;
+    t_lo = padded_div<R>(Gc - eg, L.d_len);
     t_lo -= pad_of<R>(fabs(t_lo));
-    R t_hi = (Gc + eg) / L.d_len;
+    R t_hi = padded_div<R>(Gc + eg, L.d_len);
     t_hi += pad_of<R>(fabs(t_hi));
     return t_lo < t_max && t_hi > t_min;
 }
