@@ -661,11 +661,13 @@ template <class R> __device__ void normal_texel_form(const Leaf<R> &L, MForm<R> 
 }
 
 // sub_square_bound (displaced_intersector.cpp:266-290) on the sub-square
-// with centre (a, b) and half-width rho (texel units).
+// with centre (a, b) and half-width rho (texel units). `inside`: the whole
+// texel passes square_inside, so no sub-square of it is outside.
 template <class R, bool CN>
-__device__ bool sub_square_bound(const TriRecord &T, const Leaf<R> &L, R a, R b, R rho, R t_min, R t_max, R &t_lo)
+__device__ bool sub_square_bound(const TriRecord &T, const Leaf<R> &L, R a, R b, R rho, R t_min, R t_max, R &t_lo,
+                                 bool inside)
 {
-    if (square_outside<R>(T, L.slo + L.w * a, L.tlo + L.w * b, L.w * rho))
+    if (!inside && square_outside<R>(T, L.slo + L.w * a, L.tlo + L.w * b, L.w * rho))
         return false;
     // h over the sub-square: affine up to the twist (rho eu)(rho ev).
     R hc = ((L.h00 + L.ha * a) + L.hb * b) + L.tw * a * b;
@@ -853,6 +855,9 @@ __device__ bool search_leaf(const TriRecord &T, const Leaf<R> &L, R t_min, R &t_
     int sp = 0;
     bool found = false;
     uint32_t code = 0; // depth 0, the whole texel
+    // A texel inside the triangle's domain with rounding room to spare
+    // skips its sub-squares' domain tests (each would be false).
+    const bool inside = square_inside<R>(T, L.slo + (R)0.5 * L.w, L.tlo + (R)0.5 * L.w, (R)0.5 * L.w);
     while (true) {
         int depth = code >> 10, qa = (code >> 5) & 31, qb = code & 31;
         if (depth == SUBDIV_DEPTH) {
@@ -878,7 +883,7 @@ __device__ bool search_leaf(const TriRecord &T, const Leaf<R> &L, R t_min, R &t_
                 ++st.sub;
                 R tl;
                 bool keep = sub_square_bound<R, CN>(T, L, ((R)ca + (R)0.5) * inv, ((R)cb + (R)0.5) * inv, rho, t_min,
-                                               t_max, tl);
+                                               t_max, tl, inside);
                 key[q] = keep ? tl : (R)INFINITY;
             }
             // Sort the four by key (a network that keeps equal keys in q order).
