@@ -873,7 +873,7 @@ __device__ bool search_leaf(const TriRecord &T, const Leaf<R> &L, R t_min, R &t_
             }
         } else {
             int cd = depth + 1;
-            R inv = (R)1 / (R)(1 << cd);
+            R inv = ldexp((R)1, -cd); // 1 / 2^cd, exactly
             R rho = (R)0.5 * inv;
             R key[4];
             uint32_t cc[4];
