@@ -277,6 +277,8 @@ bool fill_record(const HostTri &h, int n_leaf, TriRecord &R)
             R.mk_d[r] = k;
             R.mc[r] = (float)c;
             R.mk[r] = f_up((k + std::abs(c - (double)R.mc[r])) * (1.0 + 0x1p-50));
+            // the double sum's rounding is inside the 2^-50 bump
+            R.mck[r] = f_up((std::abs((double)R.mc[r]) + (double)R.mk[r]) * (1.0 + 0x1p-50));
         }
     }
     for (int q = 0; q < 3; ++q) {

@@ -92,7 +92,8 @@ struct TriRecord
     float f_T[9];
     double residual_tolerance; // 1e-10 x mean edge (displaced_intersector.cpp:150)
     float aabb_lo[3], aabb_hi[3];
-    float pad_[2];
+    float mck[3]; // |mc| + mk per axis, rounded up
+    float pad_[1];
 };
 
 } // namespace dmap_amd

@@ -388,11 +388,16 @@ __device__ bool cell_test(const TraceParams &p, const PyramidView &P, const TriR
         // Under a constant normal (CN) the symbol coefficients are zero and
         // these are the same values without the zero terms.
         R dev_m = CN ? m.k[q] : fabs(m.a[q]) + fabs(m.b[q]) + m.k[q];
-        R c = pc[q] + hc * m.c[q];
+        // (z: the base point's component is 0.)
+        R c = q == 2 ? hc * m.c[q] : pc[q] + hc * m.c[q];
         R a = CN ? pa[q] : pa[q] + hc * m.a[q];
         R b = CN ? pb[q] : pb[q] + hc * m.b[q];
-        R k = fabs(m.c[q]) * hr + fabs(hc) * m.k[q] + hr * dev_m;
-        R e = fabs(a) + fabs(b) + k;
+        // In fp32 under a constant normal, hr (|mc| + mk) with the sum read
+        // rounded up (TriRecord::mck): one product fewer, never smaller.
+        R k = (CN && sizeof(R) == sizeof(float)) ? hr * (R)T.mck[q] + fabs(hc) * m.k[q]
+                                                  : fabs(m.c[q]) * hr + fabs(hc) * m.k[q] + hr * dev_m;
+        // (CN: |a| + |b| is hx, hy or 0 on x, y, z.)
+        R e = CN ? (q == 2 ? k : (q == 0 ? hx : hy) + k) : fabs(a) + fabs(b) + k;
         e += pad_of<R>(fabs(c) + e);
         bc[q] = c;
         be[q] = e;
