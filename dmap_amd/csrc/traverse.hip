@@ -105,6 +105,24 @@ template <class R> __device__ inline bool clip_centre(R o, R d, R inv, R c, R e,
     return true;
 }
 
+// clip_centre for a node box whose centre c and radius e are themselves
+// rounded results: one pad of 8 unit roundoffs of |o| + 2 (|c| + e) is
+// the room for their rounding (8 of |c| + e) and for the clip's own (8 of
+// |o| + |c| + e, which needs under 5), folded into one radius.
+template <class R> __device__ inline bool clip_node_box(R o, R d, R inv, R c, R e, R &t0, R &t1)
+{
+    R E = e + pad_of<R>(fabs(o) + (R)2 * (fabs(c) + e));
+    if (d != (R)0) {
+        R tm = (c - o) * inv, tr = E * fabs(inv);
+        tr += pad_of<R>(fabs(tm) + tr);
+        t0 = nmax(t0, tm - tr);
+        t1 = nmin(t1, tm + tr);
+    } else if (fabs(o - c) > E) {
+        return false;
+    }
+    return true;
+}
+
 // Aabb3::clip (node_bounds.cpp:25-44) on one axis, widened by rounding room.
 template <class R> __device__ inline bool clip_axis(R o, R d, R inv, R lo, R hi, R &t0, R &t1)
 {
@@ -346,27 +364,26 @@ __device__ inline float4 fetch_tp(const PyramidView &P, int level, int i, int j)
     return P.tp[idx];
 }
 
+// One axis of a cell [k w, (k + 1) w] clipped to the triangle's texture
+// bounding box [lo, hi] (node_bounds.cpp:94-99), as centre and half-width.
+template <class R> __device__ inline void cell_span(int k, R w, R lo, R hi, R &c, R &h)
+{
+    R clo = (R)k * w;
+    R rlo = nmax(clo, lo), rhi = nmin(clo + w, hi);
+    c = (R)0.5 * (rlo + rhi);
+    h = nmax((R)0.5 * (rhi - rlo), (R)0);
+}
+
 // node_bound + node_interval (node_bounds.cpp:83-109, 111-211) for cell
-// (level, ci, cj) with its min-max pair mm and, when the slab is read, its
-// Taylor quad tp. Returns whether the ray can meet the surface over the
-// cell within [t_min, t_max].
+// (level, ci, cj), clipped to the triangle's texture bounding box as centre
+// (cx, cy) and half-widths (hx, hy) (cell_span), with its min-max pair mm
+// and, when the slab is read, its Taylor quad tp. Returns whether the ray
+// can meet the surface over the cell within [t_min, t_max].
 template <class R, bool CN>
 __device__ bool cell_test(const TraceParams &p, const PyramidView &P, const TriRecord &T, const Walk<R> &wk, int level,
-                          int ci, int cj, float2 mm, float4 tp, R t_min, R t_max)
+                          int ci, int cj, R cx, R cy, R hx, R hy, float2 mm, float4 tp, R t_min, R t_max)
 {
     R w = ldexp((R)1, level - (__ffs(P.n_leaf) - 1)); // 2^level / n_leaf, n_leaf a power of two
-    R clo_x = (R)ci * w, clo_y = (R)cj * w;
-    // The cell clipped to the triangle's texture bounding box (node_bounds.cpp:94-99).
-    R rlo_x = nmax(clo_x, (R)T.dlo[0]), rlo_y = nmax(clo_y, (R)T.dlo[1]);
-    R rhi_x = nmin(clo_x + w, (R)T.dhi[0]), rhi_y = nmin(clo_y + w, (R)T.dhi[1]);
-    if (sizeof(R) == 8) {
-        rlo_x = fmax(clo_x, (R)T.dlo_d[0]);
-        rlo_y = fmax(clo_y, (R)T.dlo_d[1]);
-        rhi_x = fmin(clo_x + w, (R)T.dhi_d[0]);
-        rhi_y = fmin(clo_y + w, (R)T.dhi_d[1]);
-    }
-    R cx = (R)0.5 * (rlo_x + rhi_x), cy = (R)0.5 * (rlo_y + rhi_y);
-    R hx = nmax((R)0.5 * (rhi_x - rlo_x), (R)0), hy = nmax((R)0.5 * (rhi_y - rlo_y), (R)0);
 
     MForm<R> m;
     if (CN || T.const_normal) {
@@ -398,8 +415,7 @@ __device__ bool cell_test(const TraceParams &p, const PyramidView &P, const TriR
                                                   : fabs(m.c[q]) * hr + fabs(hc) * m.k[q] + hr * dev_m;
         // (CN: |a| + |b| is hx, hy or 0 on x, y, z.)
         R e = CN ? (q == 2 ? k : (q == 0 ? hx : hy) + k) : fabs(a) + fabs(b) + k;
-        e += pad_of<R>(fabs(c) + e);
-        bc[q] = c;
+        bc[q] = c; // unpadded: clip_node_box adds the rounding room of c and e
         be[q] = e;
     }
 
@@ -411,12 +427,12 @@ __device__ bool cell_test(const TraceParams &p, const PyramidView &P, const TriR
     // node_bounds.cpp:137-179): both start from [t_min, t_max], so the two
     // are clipped once and shared.
     R t0 = t_min, t1 = t_max;
-    if (!clip_centre<R>(wk.o[0], wk.d[0], wk.inv[0], bc[0], be[0], t0, t1) ||
-        !clip_centre<R>(wk.o[1], wk.d[1], wk.inv[1], bc[1], be[1], t0, t1))
+    if (!clip_node_box<R>(wk.o[0], wk.d[0], wk.inv[0], bc[0], be[0], t0, t1) ||
+        !clip_node_box<R>(wk.o[1], wk.d[1], wk.inv[1], bc[1], be[1], t0, t1))
         return false;
     R b0 = t0, b1 = t1;
     if (want_box) {
-        if (!clip_centre<R>(wk.o[2], wk.d[2], wk.inv[2], bc[2], be[2], b0, b1) || !close_interval<R>(b0, b1))
+        if (!clip_node_box<R>(wk.o[2], wk.d[2], wk.inv[2], bc[2], be[2], b0, b1) || !close_interval<R>(b0, b1))
             return false;
         if (!want_slab)
             return true;
@@ -1167,6 +1183,14 @@ __device__ uint32_t expand(const TraceParams &p, const PyramidView &P, const Tri
     }
     R w = ldexp((R)1, level - (__ffs(P.n_leaf) - 1)); // 2^level / n_leaf, n_leaf a power of two
     const bool inside = square_inside<R>(T, ((R)bi + (R)1) * w, ((R)bj + (R)1) * w, w);
+    // The block's two columns and two rows, clipped once for its four cells.
+    const R blo_x = sizeof(R) == 8 ? (R)T.dlo_d[0] : (R)T.dlo[0], bhi_x = sizeof(R) == 8 ? (R)T.dhi_d[0] : (R)T.dhi[0];
+    const R blo_y = sizeof(R) == 8 ? (R)T.dlo_d[1] : (R)T.dlo[1], bhi_y = sizeof(R) == 8 ? (R)T.dhi_d[1] : (R)T.dhi[1];
+    R sx_c[2], sx_h[2], sy_c[2], sy_h[2];
+    for (int k = 0; k < 2; ++k) {
+        cell_span<R>(bi + k, w, blo_x, bhi_x, sx_c[k], sx_h[k]);
+        cell_span<R>(bj + k, w, blo_y, bhi_y, sy_c[k], sy_h[k]);
+    }
     uint32_t mask = 0;
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
@@ -1178,7 +1202,8 @@ __device__ uint32_t expand(const TraceParams &p, const PyramidView &P, const Tri
             continue;
         ++node_tests;
         float4 tp = slab ? g.tp[q] : make_float4(0.f, 0.f, 0.f, 0.f);
-        if (cell_test<R, CN>(p, P, T, wk, level, ci, cj, group_mm(g, q), tp, t_min, t_max))
+        if (cell_test<R, CN>(p, P, T, wk, level, ci, cj, sx_c[io], sy_c[jo], sx_h[io], sy_h[jo], group_mm(g, q), tp,
+                             t_min, t_max))
             mask |= 1u << (((io ^ wk.si) << 1) | (jo ^ wk.sj));
     }
     return mask;
