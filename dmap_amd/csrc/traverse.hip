@@ -158,14 +158,9 @@ template <class R> __device__ inline bool close_interval(R &t0, R &t1)
     return true;
 }
 
-struct Iv
-{
-    double lo, hi;
-};
 template <class R> struct IvR
 {
     R lo, hi;
-    __device__ R mag() const { return nmax(fabs(lo), fabs(hi)); }
 };
 
 // The normalized normal's affine form in tangent space over one cell:
@@ -459,24 +454,27 @@ __device__ bool cell_test(const TraceParams &p, const PyramidView &P, const TriR
     const R xr = CN ? m.k[0] : fabs(m.a[0]) + fabs(m.b[0]) + m.k[0],
             yr = CN ? m.k[1] : fabs(m.a[1]) + fabs(m.b[1]) + m.k[1],
             zr = CN ? m.k[2] : fabs(m.a[2]) + fabs(m.b[2]) + m.k[2];
-    const IvR<R> mx{xc - xr, xc + xr}, my{yc - yr, yc + yr}, mz{zc - zr, zc + zr};
+    // Magnitudes: the largest |value| of each form, |c| + r (in fp32 under a
+    // constant normal the per-triangle sums rounded up, TriRecord::mck).
+    const bool pre = CN && sizeof(R) == sizeof(float);
+    const R xm = pre ? (R)T.mck[0] : fabs(xc) + xr, ym = pre ? (R)T.mck[1] : fabs(yc) + yr,
+            zm = pre ? (R)T.mck[2] : fabs(zc) + zr;
     const R Ac = (R)1 + gu * xc + gv * yc, Ar = fabs(gu) * xr + fabs(gv) * yr;
     const R Bc = -gu * zc, Br = fabs(gu) * zr, Cc = -gv * zc, Cr = fabs(gv) * zr;
     R dconst = -(h0 - gu * ucx - gv * ucy);
     const R Dc = dconst * zc, Dr = fabs(dconst) * zr;
-    R Rr = rr * mz.mag();
+    R Rr = rr * zm;
     const R F0c = anchor[2] * Ac + anchor[0] * Bc + anchor[1] * Cc + Dc;
     const R F0r = fabs(anchor[2]) * Ar + fabs(anchor[0]) * Br + fabs(anchor[1]) * Cr + Dr;
     const R F1c = wk.d[2] * Ac + wk.d[0] * Bc + wk.d[1] * Cc;
     const R F1r = fabs(wk.d[2]) * Ar + fabs(wk.d[0]) * Br + fabs(wk.d[1]) * Cr;
     const IvR<R> F0{F0c - F0r, F0c + F0r}, F1{F1c - F1r, F1c + F1r};
     // Rounding room of everything above, folded into the remainder R.
-    R magA = (R)1 + fabs(gu) * mx.mag() + fabs(gv) * my.mag(), magB = fabs(gu) * mz.mag(),
-      magC = fabs(gv) * mz.mag();
-    R magD = (fabs(h0) + fabs(gu * ucx) + fabs(gv * ucy)) * mz.mag();
+    R magA = (R)1 + fabs(gu) * xm + fabs(gv) * ym, magB = fabs(gu) * zm, magC = fabs(gv) * zm;
+    R magD = (fabs(h0) + fabs(gu * ucx) + fabs(gv * ucy)) * zm;
     R err = (fabs(anchor[2]) + span * fabs(wk.d[2])) * magA + (fabs(anchor[0]) + span * fabs(wk.d[0])) * magB +
             (fabs(anchor[1]) + span * fabs(wk.d[1])) * magC + magD +
-            (wk.omag + fabs(t0) * wk.dmag) * (magA + magB + magC) + Rr + F0.mag();
+            (wk.omag + fabs(t0) * wk.dmag) * (magA + magB + magC) + Rr + (fabs(F0c) + F0r);
     Rr += pad_of<R>(err);
     R s0 = 0, s1 = span;
     if (F1.lo > (R)0)
