@@ -1161,7 +1161,7 @@ __device__ inline float4 group_tp(const Group &g, int q)
 // child order of cell_next (displaced_intersector.cpp:99-121).
 template <class R, bool CN>
 __device__ uint32_t expand(const TraceParams &p, const PyramidView &P, const TriRecord &T, const Walk<R> &wk, int level,
-                           int bi, int bj, bool aligned, uint32_t valid, R t_min, R t_max, int &node_tests)
+                           int bi, int bj, bool aligned, uint32_t valid, bool &inside, R t_min, R t_max, int &node_tests)
 {
     bool slab = p.bound != 0 && level <= p.slab_max_level;
     Group g;
@@ -1180,7 +1180,10 @@ __device__ uint32_t expand(const TraceParams &p, const PyramidView &P, const Tri
         }
     }
     R w = ldexp((R)1, level - (__ffs(P.n_leaf) - 1)); // 2^level / n_leaf, n_leaf a power of two
-    const bool inside = square_inside<R>(T, ((R)bi + (R)1) * w, ((R)bj + (R)1) * w, w);
+    // `inside` comes in set when an ancestor block was inside (then so is
+    // every block below it); otherwise it is tested here.
+    if (!inside)
+        inside = square_inside<R>(T, ((R)bi + (R)1) * w, ((R)bj + (R)1) * w, w);
     // The block's two columns and two rows, clipped once for its four cells.
     const R blo_x = sizeof(R) == 8 ? (R)T.dlo_d[0] : (R)T.dlo[0], bhi_x = sizeof(R) == 8 ? (R)T.dhi_d[0] : (R)T.dhi[0];
     const R blo_y = sizeof(R) == 8 ? (R)T.dlo_d[1] : (R)T.dlo[1], bhi_y = sizeof(R) == 8 ? (R)T.dhi_d[1] : (R)T.dhi[1];
@@ -1301,6 +1304,9 @@ k_trace(TraceParams p)
     int root_level = 0, rb_i = 0, rb_j = 0;
     int lvl = 0, pi = 0, pj = 0; // the group being walked: the cells at lvl below (pi, pj), or the root block
     uint64_t trail = 0;
+    // The level of the block on the current path found inside the
+    // triangle's domain (its cells and all below them are inside), or -1.
+    int in_lvl = -1;
     int ex_level = 0, ex_i = 0, ex_j = 0; // the block to expand
     bool ex_aligned = true;
     uint32_t ex_valid = 15;
@@ -1415,6 +1421,8 @@ k_trace(TraceParams p)
                 }
                 int up = (__ffsll((unsigned long long)rest) - 1) >> 2;
                 lvl += up;
+                if (lvl > in_lvl)
+                    in_lvl = -1; // the walk has left the inside block
                 pi >>= up;
                 pj >>= up;
                 uint32_t m = (uint32_t)(trail >> (4 * lvl)) & 15u;
@@ -1491,12 +1499,16 @@ k_trace(TraceParams p)
             ex_j = rb_j;
             ex_aligned = false;
             ex_valid = rv;
+            in_lvl = -1;
             state = ST_EXPAND;
         }
         if (state == ST_EXPAND) {
             int tests = 0;
-            uint32_t mask =
-                expand<R, CN>(p, DMAP_OBJ.pyr, *T, wk, ex_level, ex_i, ex_j, ex_aligned, ex_valid, t_min, t_best, tests);
+            bool inside = ex_level < in_lvl;
+            uint32_t mask = expand<R, CN>(p, DMAP_OBJ.pyr, *T, wk, ex_level, ex_i, ex_j, ex_aligned, ex_valid, inside,
+                                          t_min, t_best, tests);
+            if (inside && in_lvl < 0)
+                in_lvl = ex_level;
             if (STATS)
                 st.node += tests;
             if (!ex_aligned) {
