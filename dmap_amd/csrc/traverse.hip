@@ -875,8 +875,11 @@ __device__ bool search_leaf(const TriRecord &T, const Leaf<R> &L, R t_min, R &t_
     bool found = false;
     uint32_t code = 0; // depth 0, the whole texel
     // A texel inside the triangle's domain with rounding room to spare
-    // skips its sub-squares' domain tests (each would be false).
-    const bool inside = square_inside<R>(T, L.slo + (R)0.5 * L.w, L.tlo + (R)0.5 * L.w, (R)0.5 * L.w);
+    // skips its sub-squares' domain tests (each would be false). Only in
+    // fp32: the fp64 kernel runs at 2 waves with spills already, and the
+    // flag's register made its leaf search 24% slower.
+    const bool inside = sizeof(R) == sizeof(float) &&
+                        square_inside<R>(T, L.slo + (R)0.5 * L.w, L.tlo + (R)0.5 * L.w, (R)0.5 * L.w);
     while (true) {
         int depth = code >> 10, qa = (code >> 5) & 31, qb = code & 31;
         if (depth == SUBDIV_DEPTH) {
