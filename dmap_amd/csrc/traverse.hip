@@ -1264,6 +1264,14 @@ k_trace(TraceParams p)
     // per-triangle bookkeeping.
     const bool lds_bvh = !MULTI && p.obj.bvh_leaves <= LDS_BVH_LEAVES;
     unsigned char *bvh_lds = (unsigned char *)(s_code + STACK * blockDim.x);
+    // Multi-request launches: the step's request starts, staged in LDS
+    // (after the stacks) for the refill's binary search.
+    int64_t *s_seg_start = (int64_t *)bvh_lds;
+    if (MULTI) {
+        for (int q = threadIdx.x; q < p.step->n_seg; q += blockDim.x)
+            s_seg_start[q] = p.step->seg[q].start;
+        __syncthreads();
+    }
     if (lds_bvh) {
         uint32_t *dst = (uint32_t *)bvh_lds;
         const uint32_t *src = (const uint32_t *)p.obj.bvh;
@@ -1367,7 +1375,7 @@ k_trace(TraceParams p)
                             int lo = 0, hi = p.step->n_seg - 1;
                             while (lo < hi) {
                                 int mid = (lo + hi + 1) >> 1;
-                                if (p.step->seg[mid].start <= slot)
+                                if (s_seg_start[mid] <= slot)
                                     lo = mid;
                                 else
                                     hi = mid - 1;
@@ -1562,9 +1570,9 @@ k_trace(TraceParams p)
 // crossing recomputed (two-triangle leaf), then st, the chart's
 // barycentrics and ng = unit (Su x Sv) at the root
 // (displaced_intersector.cpp:357-366, 413-421).
-__device__ void finalize_hit(const TraceParams &p, const PyramidView &P, const TriRecord *tris, const RaysDev &rays,
-                             const HitsDev &H, int64_t rid, int tri, float t_raw, float a_raw, float b_raw, int hi,
-                             int hj)
+__device__ __forceinline__ void finalize_hit(const TraceParams &p, const PyramidView &P, const TriRecord *tris,
+                                             const RaysDev &rays, const HitsDev &H, int64_t rid, int tri, float t_raw,
+                                             float a_raw, float b_raw, int hi, int hj)
 {
     H.prim[rid] = tri;
     if (tri < 0) {
@@ -1665,8 +1673,8 @@ __device__ void finalize_hit(const TraceParams &p, const PyramidView &P, const T
 }
 
 // The same from the raw record the trace left in the hit arrays.
-__device__ void finalize_one(const TraceParams &p, const PyramidView &P, const TriRecord *tris, const RaysDev &rays,
-                             const HitsDev &H, int64_t rid)
+__device__ __forceinline__ void finalize_one(const TraceParams &p, const PyramidView &P, const TriRecord *tris,
+                                             const RaysDev &rays, const HitsDev &H, int64_t rid)
 {
     finalize_hit(p, P, tris, rays, H, rid, H.prim[rid], H.t[rid], H.s[rid], H.tt[rid], H.texel_i[rid],
                  H.texel_j[rid]);
@@ -1676,20 +1684,34 @@ template <bool MULTI> __global__ void __launch_bounds__(256) k_finalize(TracePar
 {
     int64_t rid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const Segment *SG = nullptr;
-    if (MULTI) {
-        // Grid-stride over the step's requests (the grid is fixed for graph replay).
+    if constexpr (MULTI) {
+        // Grid-stride over the step's requests (the grid is fixed for graph
+        // replay); the request starts are staged in LDS for the search.
         const StepDesc &S = *p.step;
+        __shared__ int64_t s_start[MAX_SEGMENTS];
+        for (int q = threadIdx.x; q < S.n_seg; q += blockDim.x)
+            s_start[q] = S.seg[q].start;
+        __syncthreads();
         for (int64_t slot = rid; slot < S.total; slot += (int64_t)gridDim.x * blockDim.x) {
             int lo = 0, hi = S.n_seg - 1;
             while (lo < hi) {
                 int mid = (lo + hi + 1) >> 1;
-                if (S.seg[mid].start <= slot)
+                if (s_start[mid] <= slot)
                     lo = mid;
                 else
                     hi = mid - 1;
             }
-            SG = &S.seg[lo];
-            finalize_one(p, SG->obj->pyr, SG->obj->tris, SG->rays, SG->hits, slot - SG->start + SG->first);
+            // Most waves lie within one request: read its record through a
+            // wave-uniform index (scalar loads) instead of lane by lane.
+            const int lo0 = __builtin_amdgcn_readfirstlane(lo);
+            if (__all(lo == lo0)) {
+                const Segment &U = S.seg[lo0];
+                const ObjectDev *uo = U.obj;
+                finalize_one(p, uo->pyr, uo->tris, U.rays, U.hits, slot - U.start + U.first);
+            } else {
+                SG = &S.seg[lo];
+                finalize_one(p, SG->obj->pyr, SG->obj->tris, SG->rays, SG->hits, slot - SG->start + SG->first);
+            }
         }
         return;
     }
@@ -2223,7 +2245,7 @@ hipError_t launch_trace_multi(const TraceParams &p, bool const_normal, hipStream
 {
     // As many waves as the persistent single-object launch.
     unsigned blocks = (unsigned)cu_count() * 4 * (BLOCK / MULTI_BLOCK);
-    const size_t lds = trace_lds_bytes<float>(MULTI_BLOCK);
+    const size_t lds = trace_lds_bytes<float>(MULTI_BLOCK) + sizeof(int64_t) * MAX_SEGMENTS;
     hipLaunchKernelGGL(k_queue_reset, dim3(1), dim3(1), 0, stream, p.queue);
     // A serving step holds objects of one kind only (constant normals or
     // interpolated ones), so it runs the same specialisation a single-object
