@@ -60,7 +60,7 @@ def main():
     ap.add_argument("--serve-load", type=float, default=0.8,
                     help="serving arrival rate as a fraction of the plain kernel's Mrays/s on the same rays")
     ap.add_argument("--serve-slots", type=int, default=3)
-    ap.add_argument("--serve-step", type=int, default=1 << 19, help="rays per serving step")
+    ap.add_argument("--serve-step", type=int, default=1 << 20, help="rays per serving step")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint timing section")
     ap.add_argument("--no-pathtrace", action="store_true", help="skip the path-tracing timing section")
     ap.add_argument("--no-fp64", action="store_true", help="skip the fp64 device run and the fp32 error report")

@@ -421,7 +421,7 @@ class Server:
     (dmap_server_*): submit requests, poll completions with latencies,
     read and release their hits."""
 
-    def __init__(self, slots=3, step_rays=1 << 19, hit_pool_bytes=4 << 30, ctx=None):
+    def __init__(self, slots=3, step_rays=1 << 20, hit_pool_bytes=4 << 30, ctx=None):
         L = load_library()
         c = ctypes
         vp = c.c_void_p
