@@ -71,8 +71,8 @@ def main():
         calibrate_copy(args.calibrate_copy)
         return
     if args.gpus != 1:
-        print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus}); "
-              "rays shard by index range with no exchange, see DESIGN.md", file=sys.stderr)
+        print(f"bench.py measures one GPU (asked for --gpus {args.gpus}): the library has no multi-GPU path; "
+              "rays would shard by index range with no exchange step", file=sys.stderr)
         sys.exit(2)
 
     import torch
