@@ -133,3 +133,18 @@ def test_both_normal_kinds(small_scenes, torch_gpu):
     mix = _run(srv, jobs)
     for q, (obj, rays) in enumerate(jobs):
         assert _same(mix[q], _host(obj.intersect(rays))), f"request {q}"
+
+
+def test_full_steps_of_tiny_requests(small_scenes):
+    """300 requests of 8-40 rays in 1M-ray steps: a step fills up to its
+    256-request limit (the step kernels stage the request starts in LDS and
+    search them per slot), and every request's hits equal the
+    single-request entry point's."""
+    import dmap_amd
+    scn = small_scenes
+    reqs = serving.make_requests(scn, 300, seed=23, min_rays=8, max_rays=40)
+    jobs = [(scn[k].obj, r) for k, r in reqs]
+    srv = dmap_amd.Server(slots=1, step_rays=1 << 20, hit_pool_bytes=64 << 20)
+    mix = _run(srv, jobs)
+    for q, (obj, rays) in enumerate(jobs):
+        assert _same(mix[q], _host(obj.intersect(rays))), f"request {q} differs"
