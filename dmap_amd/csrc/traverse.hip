@@ -1058,7 +1058,8 @@ __device__ __forceinline__ int depth_of(uint32_t node) { return 31 - __clz(node)
 // nearest, displaced_intersector.cpp:578-599).
 template <class R>
 __device__ __forceinline__ int bvh_next(const BvhNode *N, const int *leaf, uint32_t P, uint32_t &cur,
-                                        uint32_t &trail, const R o[3], const R d[3], const R inv[3], R t_min, R t_max)
+                                        uint32_t &trail, const R o[3], const R d[3], const R inv[3], R t_min, R t_max,
+                                        bool narrowed)
 {
     if (cur == BVH_DONE)
         return -1;
@@ -1085,7 +1086,9 @@ __device__ __forceinline__ int bvh_next(const BvhNode *N, const int *leaf, uint3
             uint32_t far = (cur >> (dc - L - 1)) ^ 1u;
             cur = far;
             int c = (int)(far & 1u);
-            if (!ray_box_cr<R>(o, d, inv, N[anc].ctr[c], N[anc].rad[c], t_min, t_max))
+            // The far child passed its box test when it was pushed; only a
+            // t_max narrowed since (by a hit) can fail it now.
+            if (narrowed && !ray_box_cr<R>(o, d, inv, N[anc].ctr[c], N[anc].rad[c], t_min, t_max))
                 continue;
             pop = false;
         }
@@ -1469,10 +1472,11 @@ k_trace(TraceParams p)
             } else {
                 if (!MULTI && lds_bvh)
                     k = bvh_next<R>((const BvhNode *)bvh_lds, (const int *)(bvh_lds + sizeof(BvhNode) * LDS_BVH_LEAVES),
-                                    (uint32_t)p.obj.bvh_leaves, bvh_cur, bvh_trail, o_f, d_f, inv_f, t_min, t_best);
+                                    (uint32_t)p.obj.bvh_leaves, bvh_cur, bvh_trail, o_f, d_f, inv_f, t_min, t_best,
+                                    hit_tri >= 0);
                 else
                     k = bvh_next<R>(DMAP_OBJ.bvh, DMAP_OBJ.bvh_leaf, (uint32_t)DMAP_OBJ.bvh_leaves, bvh_cur,
-                                    bvh_trail, o_f, d_f, inv_f, t_min, t_best);
+                                    bvh_trail, o_f, d_f, inv_f, t_min, t_best, hit_tri >= 0);
             }
             if (k < 0) {
                 state = ST_DONE;
