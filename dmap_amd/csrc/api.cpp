@@ -83,7 +83,11 @@ AF mul(const AF &x, const AF &r)
 AF scale(double s, const AF &x) { return {x.c * s, x.a * s, x.b * s, x.k * std::abs(s)}; }
 AF add(const AF &x, const AF &y) { return {x.c + y.c, x.a + y.a, x.b + y.b, x.k + y.k}; }
 
-// AffineVec3::normalize (affine_form.cpp:64-95).
+// AffineVec3::normalize (affine_form.cpp:64-95). The squared length's
+// remainder bounds the cross term 2 sum_q c_q k_q e_q by 2 sum_q |c_q| k_q,
+// its maximum over independent e_q in [-1, 1]; affine_form.cpp:83 writes
+// 2 |c . k|, which equals it when every c_q k_q has one sign and is too
+// small otherwise. Every caller here passes k = 0, where both are 0.
 void normalize(AF N[3])
 {
     double dev[3], cc = 0, cu = 0, cv = 0, ck = 0, dd = 0;
@@ -95,10 +99,10 @@ void normalize(AF N[3])
         cc += N[q].c * N[q].c;
         cu += N[q].c * N[q].a;
         cv += N[q].c * N[q].b;
-        ck += N[q].c * N[q].k;
+        ck += std::abs(N[q].c) * N[q].k;
     }
     double offset = 0.5 * dd;
-    AF l{cc + offset, 2.0 * cu, 2.0 * cv, 2.0 * std::abs(ck) + offset};
+    AF l{cc + offset, 2.0 * cu, 2.0 * cv, 2.0 * ck + offset};
     double lo = l.c - l.dev(), hi = l.c + l.dev();
     if (lo <= 0.0) {
         for (int q = 0; q < 3; ++q)

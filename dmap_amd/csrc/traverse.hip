@@ -189,7 +189,11 @@ template <class R> __device__ inline AF<R> af_mul(const AF<R> &x, const AF<R> &r
 }
 
 // AffineVec3::normalize (affine_form.cpp:78-95: sq_length, Min-Range
-// rsqrt, product) with the rounding room of the float path.
+// rsqrt, product) with the rounding room of the float path. The input
+// carries that room as remainders k_q, so the squared length bounds its
+// cross term 2 sum_q c_q k_q e_q by 2 sum_q |c_q| k_q (affine_form.cpp:83
+// writes 2 |c . k|, a bound only when every c_q k_q has one sign; the
+// reference's own inputs have k = 0).
 template <class R> __device__ void affine_normalize(AF<R> N[3])
 {
     R dev[3], cc = 0, cu = 0, cv = 0, ck = 0, dd = 0;
@@ -198,11 +202,11 @@ template <class R> __device__ void affine_normalize(AF<R> N[3])
         cc += N[q].c * N[q].c;
         cu += N[q].c * N[q].a;
         cv += N[q].c * N[q].b;
-        ck += N[q].c * N[q].k;
+        ck += fabs(N[q].c) * N[q].k; // the cross term's bound over independent remainders
         dd += dev[q] * dev[q];
     }
     R offset = (R)0.5 * dd;
-    AF<R> l{cc + offset, (R)2 * cu, (R)2 * cv, (R)2 * fabs(ck) + offset};
+    AF<R> l{cc + offset, (R)2 * cu, (R)2 * cv, (R)2 * ck + offset};
     l.k += pad_of<R>(cc + fabs(l.a) + fabs(l.b) + l.k);
     R lo = l.c - af_dev(l), hi = l.c + af_dev(l);
     if (lo <= (R)0) {

@@ -17,7 +17,13 @@ Parity pins (each function of the restatement is checked by at least one):
     through them inside every interval;
   * walk + leaves: brute force over all texels (two-triangle leaf vs
     Moller-Trumbore on the texel mesh; Newton leaf vs the bilinear-patch
-    closed form), never-miss on aimed rays, bound-mode invariance, flat map.
+    closed form), never-miss on aimed rays, bound-mode invariance, flat map;
+  * the general-normal and general-chart branch (interpolated normals,
+    rotated charts, tilted frames; tests/test_oracle_general.py): the
+    surface and tangent frame against their barycentric definition, Min-Range
+    normalize against dense sampling, node bounds against dense surface
+    points, never-miss, the Newton residual, a texel-aligned fine mesh, and
+    bound-mode invariance.
 """
 import ctypes
 import os
@@ -92,6 +98,8 @@ def _declare(L):
     L.orc_classify_square.argtypes = [_D, c.c_double, c.c_double, c.c_double]
     L.orc_surface_point.argtypes = [_VP, c.c_int, c.c_double, c.c_double, _D]
     L.orc_hit_record.argtypes = [_VP, c.c_int, c.c_double, c.c_double, _D, _D, c.c_int, _D]
+    L.orc_affine_normalize.argtypes = [_D, _D]
+    L.orc_tangent_frame.argtypes = [_VP, c.c_int, _D, _D]
     L.orc_scene_adjoint.argtypes = [_VP, c.c_int64, _D, _D, _D, _D, _D, _D, c.c_int, _D, _D, _D]
 
 
@@ -136,6 +144,14 @@ def close_tile(texels, repeat=True):
     out = np.empty((n + 1, n + 1), dtype=np.float64)
     lib().orc_close_tile(_p(T), n, 1 if repeat else 0, _p(out))
     return out
+
+
+def affine_normalize(form):
+    """AffineVec3::normalize on a (3, 4) array of rows (c, a, b, k)."""
+    f = _f64(form).reshape(12)
+    out = np.empty(12)
+    lib().orc_affine_normalize(_p(f), _p(out))
+    return out.reshape(3, 4)
 
 
 def classify_square(t6, cx, cy, s):
@@ -270,6 +286,12 @@ class Scene:
         ok = lib().orc_node_interval(self.handle, k, level, i, j, _p(_f64(o)), _p(_f64(d)), tmin, tmax,
                                      BOUND[mode], _p(t01))
         return bool(ok), t01
+
+    def tangent_frame(self, k):
+        """(origin q0, to_tangent 3x3) of triangle k."""
+        o, m = np.zeros(3), np.zeros(9)
+        lib().orc_tangent_frame(self.handle, int(k), _p(o), _p(m))
+        return o, m.reshape(3, 3)
 
     def displaced_position(self, k, s, t):
         S = np.zeros(3)

@@ -1727,6 +1727,35 @@ void orc_field_eval(void *scene, int64_t n, const double *xy, double *h, double 
     }
 }
 
+// AffineVec3::normalize (affine_form.cpp:87-95) on one affine vector. in12
+// and out12 hold x, y, z as (c, a, b, k) each: c + a eu + b ev +- k.
+void orc_affine_normalize(const double *in12, double *out12)
+{
+    AV3 v{AF{in12[0], in12[1], in12[2], in12[3]}, AF{in12[4], in12[5], in12[6], in12[7]},
+          AF{in12[8], in12[9], in12[10], in12[11]}};
+    normalize(v);
+    const AF *f[3] = {&v.x, &v.y, &v.z};
+    for (int a = 0; a < 3; ++a) {
+        out12[4 * a] = f[a]->c;
+        out12[4 * a + 1] = f[a]->a;
+        out12[4 * a + 2] = f[a]->b;
+        out12[4 * a + 3] = f[a]->k;
+    }
+}
+
+// tangent_frame (node_bounds.cpp:13-23) of triangle k: origin (3) and the
+// row-major to_tangent matrix (9).
+void orc_tangent_frame(void *scene, int k, double *origin3, double *to_tangent9)
+{
+    const Frame &f = ((const Scene *)scene)->its[k].frame;
+    origin3[0] = f.origin.x;
+    origin3[1] = f.origin.y;
+    origin3[2] = f.origin.z;
+    for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c)
+            to_tangent9[3 * r + c] = f.to_tangent.m[r][c];
+}
+
 int orc_classify_square(const double *t6, double cx, double cy, double s)
 {
     UvTri u;

@@ -29,7 +29,7 @@ int orc_pyramid_levels(int nodes);
 /* Total cells over all levels: sum over k of (n >> k)^2. */
 int64_t orc_pyramid_cells(int nodes);
 
-/* TaylorPyramid(values, nodes, scale, build, offset) (taylor_pyramid.cpp:324-360).
+/* TaylorPyramid(values, nodes, scale, build, offset) (taylor_pyramid.cpp:215-251).
  * build: 0 fold, 1 direct. out receives, level by level from the leaves up,
  * the 8 channels of that level, each an m*m row-major [j*m + i] array.
  * Returns the level count, or -1 for a node grid that is not 2^L + 1. */
@@ -82,12 +82,12 @@ int orc_newton_from(void *scene, int k, const double *o, const double *d, double
 int orc_leaf_two_triangle(void *scene, int k, const double *o, const double *d, double tmin, double tmax, int64_t i,
                           int64_t j, double *hit10);
 
-/* node_bound (node_bounds.cpp:192-218): box lo/hi (6), m intervals (6:
+/* node_bound (node_bounds.cpp:83-109): box lo/hi (6), m intervals (6:
  * lo x, hi x, lo y, hi y, lo z, hi z), centre (2), rect lo/hi (4), the
  * node's 8 channels. */
 void orc_node_bound(void *scene, int k, int level, int64_t i, int64_t j, double *box6, double *m6, double *centre2,
                     double *rect4, double *node8);
-/* node_interval and the three tests (node_bounds.cpp:239-320) on a ray in
+/* node_interval and the three tests (node_bounds.cpp:130-211) on a ray in
  * object space. mode: 0 box, 1 slab, 2 both. */
 int orc_node_interval(void *scene, int k, int level, int64_t i, int64_t j, const double *o, const double *d,
                       double tmin, double tmax, int mode, double *t01);
@@ -101,6 +101,14 @@ void orc_surface_normal(void *scene, int k, double s, double t, double *n3);
 void orc_surface_points(void *scene, int64_t n, const int32_t *tri, const double *st, double *S3, double *n3);
 /* The height field h and its gradient at texture coordinates (x, y). */
 void orc_field_eval(void *scene, int64_t n, const double *xy, double *h, double *grad2);
+
+/* AffineVec3::normalize (affine_form.cpp:87-95; Min-Range rsqrt :64-76,
+ * sq_length :78-85) of one affine vector: in12/out12 hold x, y, z as
+ * (c, a, b, k), the form c + a eu + b ev +- k. */
+void orc_affine_normalize(const double *in12, double *out12);
+/* tangent_frame (node_bounds.cpp:13-23) of triangle k: the origin q0 and the
+ * row-major 3x3 to_tangent = [e1 e2 ng]^-1. */
+void orc_tangent_frame(void *scene, int k, double *origin3, double *to_tangent9);
 
 /* classify_square (uv_clip.cpp:27-44): 0 outside, 1 inside, 2 straddle. */
 int orc_classify_square(const double *t6, double cx, double cy, double s);
