@@ -155,8 +155,12 @@ bool make_tri(const double *pos, const double *nrm, const double *uv, HostTri &t
     return true;
 }
 
-bool fill_record(const HostTri &h, int n_leaf, TriRecord &R)
+// Fills a triangle's device record. Returns false for a degenerate frame or
+// footprint; *root_level_out receives the root level found (also when it
+// is too high for the walk, MAX_ROOT_LEVEL, which then returns false).
+bool fill_record(const HostTri &h, int n_leaf, TriRecord &R, int *root_level_out)
 {
+    *root_level_out = -1;
     std::memset(&R, 0, sizeof R);
     const V3 *vs[6] = {&h.q0, &h.e1, &h.e2, &h.m0, &h.Mu, &h.Mv};
     double *ds[6] = {R.q0, R.e1, R.e2, R.m0, R.Mu, R.Mv};
@@ -243,8 +247,11 @@ bool fill_record(const HostTri &h, int n_leaf, TriRecord &R)
         return false;
     if (std::abs(i_lo) > (1 << 28) || std::abs(j_lo) > (1 << 28))
         return false;
-    // The walk keeps four bits per level below the root in 64 bits.
-    if (level > 16)
+    // The walk keeps four trail bits for each level from the root block's
+    // down to level 0 in one 64-bit word: at most 16 levels, so a root
+    // level of at most MAX_ROOT_LEVEL = 15.
+    *root_level_out = level;
+    if (level > MAX_ROOT_LEVEL)
         return false;
     R.root_level = level;
     R.n_roots = 0;
@@ -798,8 +805,14 @@ int dmap_object_create(dmap_ctx *ctx, dmap_tile *tile, int n_tri, const double *
             delete o;
             return fail(DMAP_ERR_INVALID, "dmap_object_create: triangle %d is degenerate in the texture plane", k);
         }
-        if (!fill_record(tris[k], tile->n, o->host[k])) {
+        int root_level = -1;
+        if (!fill_record(tris[k], tile->n, o->host[k], &root_level)) {
             delete o;
+            if (root_level > MAX_ROOT_LEVEL)
+                return fail(DMAP_ERR_INVALID,
+                            "dmap_object_create: triangle %d needs root level %d (its uv span over a %d-texel tile); "
+                            "the walk supports root levels up to %d",
+                            k, root_level, tile->n, MAX_ROOT_LEVEL);
             return fail(DMAP_ERR_INVALID, "dmap_object_create: triangle %d has a degenerate frame or footprint", k);
         }
     }

@@ -63,6 +63,10 @@ __host__ __device__ inline uint32_t level_base(uint32_t m) { return (m * m - 1u)
 // TangentFrame (node_bounds.cpp:13-23), UvTriangle (uv_clip.cpp:10-25),
 // footprint_roots (descent_sampler.cpp:66-83) and the DisplacedIntersector
 // constructor (displaced_intersector.cpp:135-151).
+// The walk keeps four trail bits per level, from the root block's level down
+// to the leaves, in one 64-bit word: a triangle's root level is at most 15.
+constexpr int MAX_ROOT_LEVEL = 15;
+
 struct TriRecord
 {
     // Surface in the chart: P = q0 + s e1 + t e2, M = m0 + s Mu + t Mv.

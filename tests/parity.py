@@ -1,8 +1,9 @@
 """Comparison of GPU hits with the oracle's, for the parity tests.
 
 The bar: hit/miss agree exactly outside the stated bands; where both hit,
-t agrees to 1e-4 relative, the tile coordinates st to 1e-4 (tile units) and
-the unit normal to 1e-4. Bands where fp32 and fp64 may legitimately
+t agrees to 1e-4 relative, the tile coordinates st to 1e-4 relative
+(1e-4 x max(1, |st|) in tile units: coordinates of repeated tiles grow with
+the uv span) and the unit normal to 1e-4. Bands where fp32 and fp64 may legitimately
 disagree on hit/miss or on which root is nearest:
   * grazing: |cos(ray, normal)| < GRAZING at the nearer of the two hits;
   * domain edge: the nearer hit lies within EDGE (tile units) of its
@@ -64,7 +65,8 @@ def compare(gpu, ref, d, tri_rows, n_leaf, rel=REL):
     dt = np.zeros(len(tg))
     dt[both] = np.abs(tg[both] - tr[both]) / np.abs(tr[both])
     dst = np.zeros(len(tg))
-    dst[both] = np.abs(gpu["st"][both].astype(np.float64) - ref["st"][both]).max(axis=1)
+    dst[both] = (np.abs(gpu["st"][both].astype(np.float64) - ref["st"][both])
+                 / np.maximum(1.0, np.abs(ref["st"][both]))).max(axis=1)
     dng = np.zeros(len(tg))
     dng[both] = np.abs(gpu["ng"][both].astype(np.float64) - ref["ng"][both]).max(axis=1)
     flip = hg != hr

@@ -29,6 +29,16 @@ def rotated_chart_quad():
     return tris
 
 
+def span_quad(lo, hi):
+    """Unit quad in z = 0 whose texture coordinates run from (lo, lo) to (hi, hi):
+    a repeated tile, hi - lo tiles across the quad."""
+    tris = W.quad_plane(1, 1.0)
+    for row in tris:
+        for v in range(3):
+            row[18 + 2 * v:20 + 2 * v] = lo + (hi - lo) * row[3 * v:3 * v + 2]
+    return tris
+
+
 def rays(n, seed, lo=-0.2, hi=1.2):
     """Half from above towards the quad, half near the surface in random
     directions (the distributions of traversal_test_util.cpp:83-111)."""

@@ -76,3 +76,34 @@ def test_degenerate_rays_miss(torch_gpu):
     assert (res["prim"].cpu().numpy()[1:] == -1).all()
     empty = obj.intersect(torch.zeros((8, 0), dtype=torch.float32, device="cuda"))
     assert empty["t"].numel() == 0
+
+
+def test_root_level_15_matches_oracle(torch_gpu):
+    """The highest root level the walk supports: a 1024-texel tile repeated 35
+    times across the quad puts the roots at level 15, whose trail bits are the
+    top four of the walk's 64-bit trail."""
+    import dmap_amd
+    from tests.gpu_util import gpu_hits, oracle_hits
+    from tests.parity import compare
+    tex = np.random.default_rng(3).random((1024, 1024)).astype(np.float32)
+    tris = scenes.span_quad(10, 45)
+    tile = dmap_amd.DisplacementTile(tex, 2e-5)
+    obj = dmap_amd.ObjectIntersector(tile, tris)
+    assert [obj.triangle_info(k)["root_level"] for k in range(2)] == [15, 15]
+    rays = scenes.rays(20000, seed=41)
+    g = gpu_hits(obj, rays)
+    ref, d, _ = oracle_hits(W.close_tile(tex).astype(np.float64), 2e-5, 0.0, True, tris, rays)
+    cmp = compare(g, ref, d, tris, 1024)
+    print({k: v for k, v in cmp.items() if k != "bad_index"})
+    assert cmp["hits_ref"] > 8000
+    assert cmp["unexplained"] == 0
+
+
+def test_root_level_16_is_refused(torch_gpu):
+    """A root level above 15 would shift the walk's trail past its 64 bits:
+    object creation refuses it with the level in the message."""
+    import dmap_amd
+    tex = np.random.default_rng(3).random((1024, 1024)).astype(np.float32)
+    tile = dmap_amd.DisplacementTile(tex, 2e-5)
+    with pytest.raises(dmap_amd.DmapError, match="root level 16"):
+        dmap_amd.ObjectIntersector(tile, scenes.span_quad(10, 80))

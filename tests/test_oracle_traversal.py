@@ -267,3 +267,23 @@ def test_classify_square_against_sampling():
             assert not ins.any()
         elif cls == 1:
             assert ins.all()
+
+
+def test_root_level_grows_with_the_uv_span():
+    """footprint_roots (descent_sampler.cpp:66-83) over a repeated tile: the
+    inputs of the GPU root-level tests sit at levels 15 and 16, and the
+    oracle (whose walk has no level limit, int64 cells) still answers at 16
+    with hits on the surface."""
+    tex = np.random.default_rng(3).random((1024, 1024)).astype(np.float32)
+    nodes = W.close_tile(tex).astype(np.float64)
+    from tests import scenes
+    for (lo, hi), level in (((10, 45), 15), ((10, 80), 16)):
+        tris = scenes.span_quad(lo, hi)
+        sc = oracle.Scene(nodes, 2e-5, 0.0, True, tris)
+        assert [sc.triangle_info(k)["root_level"] for k in range(2)] == [level, level]
+        o = np.column_stack([np.linspace(0.1, 0.9, 50), np.linspace(0.2, 0.8, 50), np.full(50, 0.5)])
+        d = np.tile([0.1, -0.05, -1.0], (50, 1))
+        res = sc.intersect(o, d, threads=1)
+        assert np.isfinite(res["t"]).all()
+        S, _ = sc.surface_points(res["prim"], res["st"])
+        assert np.linalg.norm(S - (o + res["t"][:, None] * d), axis=1).max() <= 2 * sc.triangle_info(0)["residual_tolerance"]
