@@ -493,13 +493,22 @@ MAX_FILTER_PASSES = 64  # DisplacedGeometry::max_filter_passes (displaced_geomet
 FILTER_RESUME_REL = 1e-5
 
 
+def filter_resume(t):
+    """The t_min a rejected hit at t resumes from (fp32): FILTER_RESUME_REL
+    past t, and at least the next float above it. Reported hits have t >= 0
+    (newton_from rejects roots behind the origin, displaced_intersector.cpp:356),
+    so the second term only matters at t = 0."""
+    torch = _torch()
+    return torch.maximum(t + t.abs() * FILTER_RESUME_REL, torch.nextafter(t, torch.full_like(t, float("inf"))))
+
+
 def intersect_filtered(obj, rays, accept, options=None, max_passes=MAX_FILTER_PASSES):
     """Closest hit that `accept` takes: the renderer's filter-resume loop
     (intersect_callback, displaced_geometry.cpp:49-77) on a batch. `accept`
     maps a hit dict (as intersect returns, all rays) to a bool tensor (n,);
     rays whose hit it rejects are traced again with t_min raised to the
-    rejected t (the walk takes t > t_min only; here FILTER_RESUME_REL past
-    it), up to max_passes times, after which they count as misses, as
+    rejected t (the walk takes t > t_min only; here filter_resume(t) just
+    past it), up to max_passes times, after which they count as misses, as
     there. The reference counts the passes per triangle, this loop per ray
     over the object."""
     torch = _torch()
@@ -511,7 +520,7 @@ def intersect_filtered(obj, rays, accept, options=None, max_passes=MAX_FILTER_PA
         if not bool(rej.any()):
             return out
         idx = rej.nonzero().squeeze(1)
-        cur[6, idx] = out["t"][idx] * (1.0 + FILTER_RESUME_REL)
+        cur[6, idx] = filter_resume(out["t"][idx])
         again = obj.intersect(cur[:, idx].contiguous(), options=options)
         for k, v in again.items():
             out[k][..., idx] = v

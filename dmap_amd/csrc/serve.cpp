@@ -113,6 +113,15 @@ struct Request
     size_t off = 0, bytes = 0;
     dmap_hits hits{};
     bool done = false;
+    std::vector<hipEvent_t> copies; // recorded after each dmap_server_copy_hits on the caller's stream
+};
+
+// A released request's pool region whose copies may still be queued on
+// the callers' streams: it returns to the arena once all have completed.
+struct Deferred
+{
+    size_t off = 0, bytes = 0;
+    std::vector<hipEvent_t> copies;
 };
 
 struct Slot
@@ -139,6 +148,7 @@ struct dmap_server
     std::vector<Request *> pending; // submitted, rays left to hand to a step
     std::map<uint64_t, Request *> live; // submitted, not released
     std::vector<Request *> finished;    // completed, not yet polled
+    std::vector<Deferred> deferred;     // released regions waiting for their copies
     std::mutex mu;
     uint64_t submitted = 0;
     int64_t completed_rays = 0;
@@ -177,9 +187,39 @@ RaysDev rays_dev(const dmap_rays &r)
     return RaysDev{r.ox, r.oy, r.oz, r.dx, r.dy, r.dz, r.tmin, r.tmax};
 }
 
+// The pool bytes of a request of n rays, as the arena rounds them.
+size_t request_bytes(int64_t n) { return ((size_t)n * POOL_BYTES_PER_RAY + 255) & ~(size_t)255; }
+
+// Returns released regions to the arena once the copies out of them have
+// completed; a copy that failed leaves its region reclaimed as well (the
+// error surfaces on the caller's stream).
+void reclaim(dmap_server *S)
+{
+    auto &D = S->deferred;
+    for (size_t k = 0; k < D.size();) {
+        auto &ev = D[k].copies;
+        ev.erase(std::remove_if(ev.begin(), ev.end(),
+                                [](hipEvent_t e) {
+                                    if (hipEventQuery(e) == hipErrorNotReady)
+                                        return false;
+                                    (void)hipEventDestroy(e);
+                                    return true;
+                                }),
+                 ev.end());
+        if (ev.empty()) {
+            S->arena.release(D[k].off, D[k].bytes);
+            D[k] = D.back();
+            D.pop_back();
+        } else {
+            ++k;
+        }
+    }
+}
+
 // Collect finished slots; then fill every free slot with a new step.
 int pump(dmap_server *S)
 {
+    reclaim(S);
     for (Slot *sl : S->slots) {
         if (!sl->busy)
             continue;
@@ -226,7 +266,7 @@ int pump(dmap_server *S)
             if ((r->obj->const_normal ? 1 : 0) != kind)
                 continue;
             if (!r->admitted) {
-                size_t bytes = (size_t)r->n * POOL_BYTES_PER_RAY;
+                size_t bytes = request_bytes(r->n);
                 size_t off;
                 if (!S->arena.alloc(bytes, off)) {
                     if (now - r->t_submit > AGING_S)
@@ -286,7 +326,8 @@ int dmap_server_create(dmap_ctx *ctx, int n_slots, int64_t step_rays, int64_t hi
         return fail(DMAP_ERR_MEMORY, "dmap_server_create: hit pool of %lld bytes: %s", (long long)hit_pool_bytes,
                     hipGetErrorString(e));
     }
-    S->arena.init(S->arena_dev, (size_t)hit_pool_bytes);
+    // The arena hands out whole 256 B units.
+    S->arena.init(S->arena_dev, (size_t)hit_pool_bytes & ~(size_t)255);
     for (int k = 0; k < n_slots; ++k) {
         Slot *sl = new Slot;
         S->slots.push_back(sl);
@@ -340,7 +381,7 @@ int dmap_server_submit(dmap_server *S, const dmap_object *obj, const dmap_rays *
 {
     if (!S || !obj || !rays || n <= 0)
         return fail(DMAP_ERR_INVALID, "dmap_server_submit: bad argument");
-    if ((size_t)n * POOL_BYTES_PER_RAY > S->arena.bytes)
+    if (request_bytes(n) > S->arena.bytes)
         return fail(DMAP_ERR_INVALID, "dmap_server_submit: %lld rays need more than the whole hit pool",
                     (long long)n);
     std::lock_guard<std::mutex> lk(S->mu);
@@ -414,6 +455,16 @@ int dmap_server_copy_hits(dmap_server *S, uint64_t tag, const dmap_hits *dst, vo
     for (int q = 0; q < 11; ++q)
         if (to[q])
             HIP_TRY(hipMemcpyAsync(to[q], from[q], 4 * n, hipMemcpyDeviceToDevice, st));
+    // The region must outlive the copies: release defers its reuse until
+    // this event has completed.
+    hipEvent_t ev = nullptr;
+    HIP_TRY(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+    hipError_t e = hipEventRecord(ev, st);
+    if (e != hipSuccess) {
+        (void)hipEventDestroy(ev);
+        return fail(DMAP_ERR_HIP, "dmap_server_copy_hits: %s", hipGetErrorString(e));
+    }
+    it->second->copies.push_back(ev);
     return DMAP_OK;
 }
 
@@ -426,11 +477,16 @@ int dmap_server_release(dmap_server *S, uint64_t tag)
     if (it == S->live.end() || !it->second->done)
         return fail(DMAP_ERR_INVALID, "dmap_server_release: no completed request %llu", (unsigned long long)tag);
     Request *r = it->second;
-    if (r->admitted)
-        S->arena.release(r->off, r->bytes);
+    HIP_TRY(hipSetDevice(S->ctx->device));
+    if (r->admitted) {
+        if (r->copies.empty())
+            S->arena.release(r->off, r->bytes);
+        else
+            S->deferred.push_back(Deferred{r->off, r->bytes, std::move(r->copies)});
+    }
+    r->copies.clear();
     S->live.erase(it);
     delete r;
-    HIP_TRY(hipSetDevice(S->ctx->device));
     return pump(S);
 }
 
@@ -468,8 +524,18 @@ void dmap_server_destroy(dmap_server *S)
         (void)hipFree(sl->queue);
         delete sl;
     }
-    for (auto &kv : S->live)
+    for (auto &kv : S->live) {
+        for (hipEvent_t e : kv.second->copies) {
+            (void)hipEventSynchronize(e);
+            (void)hipEventDestroy(e);
+        }
         delete kv.second;
+    }
+    for (Deferred &d : S->deferred)
+        for (hipEvent_t e : d.copies) {
+            (void)hipEventSynchronize(e);
+            (void)hipEventDestroy(e);
+        }
     (void)hipFree(S->arena_dev);
     delete S;
 }

@@ -205,11 +205,17 @@ int dmap_server_submit(dmap_server *server, const dmap_object *obj, const dmap_r
 /* Completed requests since the last poll: their tags and latencies
  * (submission to completion, seconds). */
 int dmap_server_poll(dmap_server *server, int max, uint64_t *tags, double *latency_s, int *n_done);
-/* The device hit arrays of a completed request (valid until released). */
+/* The device hit arrays of a completed request (valid until released):
+ * work the caller queues on them must have completed before it releases
+ * the request, whose pool region a later step may then overwrite. */
 int dmap_server_hits(dmap_server *server, uint64_t tag, dmap_hits *hits);
 /* Copy a completed request's hits into caller-owned device arrays (NULL
- * fields are skipped), asynchronously on `stream`. */
+ * fields are skipped), asynchronously on `stream`. The request may be
+ * released right after: its pool region is reused only once the copies
+ * have completed. */
 int dmap_server_copy_hits(dmap_server *server, uint64_t tag, const dmap_hits *dst, void *stream);
+/* Ends a completed request: its pool region returns to the server once
+ * every copy queued by dmap_server_copy_hits has completed. */
 int dmap_server_release(dmap_server *server, uint64_t tag);
 /* idle = 1 when nothing is queued or in flight. */
 int dmap_server_idle(dmap_server *server, int *idle);

@@ -80,9 +80,18 @@ def test_filtered_accept_all_and_without_triangle(torch_gpu):
     assert torch.equal(occ, torch.isfinite(ref["t"]))
 
 
+def _resume_np(t):
+    """dmap_amd.filter_resume in numpy fp32."""
+    t = t.astype(np.float32)
+    return np.maximum(t + np.abs(t) * np.float32(1e-5), np.nextafter(t, np.float32(np.inf)))
+
+
 def test_filtered_texel_parity_matches_oracle_loop(torch_gpu):
+    """The filter-resume loop on the GPU against the same loop on the oracle."""
     torch = torch_gpu
     dmap_amd, tile, obj, rays, sc, tris, o, d = _setup("flat", torch, n_tex=16, res=64)
+    assert np.array_equal(_resume_np(np.array([2.0, -3.0, 0.0], np.float32)),
+                          dmap_amd.filter_resume(torch.tensor([2.0, -3.0, 0.0], device="cuda")).cpu().numpy())
     # Grazing rays cross many texels: reject hits on texels with i + j even.
     rng = np.random.default_rng(4)
     n = 4096
@@ -90,12 +99,13 @@ def test_filtered_texel_parity_matches_oracle_loop(torch_gpu):
     tg = np.column_stack([rng.uniform(0, 1, n), rng.uniform(0, 1, n), np.zeros(n)])
     dg = tg - og
     dg[:, 2] *= 0.3
+    t0 = 0.0
     rg = torch.from_numpy(W.pack_rays(og.astype(np.float32), dg.astype(np.float32))).cuda()
     accept = lambda h: (h["texel"][0] + h["texel"][1]) % 2 == 1  # noqa: E731
     filt = dmap_amd.intersect_filtered(obj, rg, accept)
     # The same loop on the oracle.
     of, df = og.astype(np.float32).astype(np.float64), dg.astype(np.float32).astype(np.float64)
-    tmin = np.zeros(n)
+    tmin = np.full(n, t0)
     t_ref = np.full(n, np.inf)
     texel_ref = np.zeros((n, 2), dtype=np.int64)
     todo = np.arange(n)
@@ -108,7 +118,7 @@ def test_filtered_texel_parity_matches_oracle_loop(torch_gpu):
         t_ref[todo[ok]] = r["t"][ok]
         texel_ref[todo[ok]] = r["texel"][ok]
         again = hit & ~ok
-        tmin[todo[again]] = (r["t"][again].astype(np.float32) * np.float32(1 + dmap_amd.FILTER_RESUME_REL))
+        tmin[todo[again]] = _resume_np(r["t"][again])
         todo = todo[again]
     tg_ = filt["t"].cpu().numpy()
     hit_g, hit_r = np.isfinite(tg_), np.isfinite(t_ref)
@@ -118,7 +128,36 @@ def test_filtered_texel_parity_matches_oracle_loop(torch_gpu):
     assert both.sum() > 150
     same_texel = np.all(filt["texel"].cpu().numpy().T[both] == texel_ref[both], axis=1)
     assert same_texel.mean() > 0.999
-    assert np.max(np.abs(tg_[both][same_texel] - t_ref[both][same_texel]) / t_ref[both][same_texel]) < 1e-4
+    assert np.max(np.abs(tg_[both][same_texel] - t_ref[both][same_texel]) / np.abs(t_ref[both][same_texel])) < 1e-4
+
     # every accepted GPU hit satisfies the filter
     tex_g = filt["texel"].cpu().numpy()
     assert np.all(((tex_g[0] + tex_g[1]) % 2 == 1)[hit_g])
+
+
+def test_no_hit_behind_the_origin(torch_gpu):
+    """A negative t_min does not make crossings behind the origin count:
+    newton_from rejects a root whose point lies behind the origin
+    (displaced_intersector.cpp:356-357), so the reported t is >= 0 and a
+    filter loop never resumes from a negative t. Rays that leave the surface
+    downwards with t_min = -4 cross it only at t in (-4, 0): no hit, on the
+    GPU and on the oracle; reversed, the same rays hit it."""
+    torch = torch_gpu
+    dmap_amd, tile, obj, rays, sc, tris, o, d = _setup("flat", torch, n_tex=16, res=64)
+    rng = np.random.default_rng(4)
+    n = 4096
+    phi = rng.uniform(0, 2 * np.pi, n)
+    og = np.column_stack([rng.uniform(0.1, 0.9, n), rng.uniform(0.1, 0.9, n), np.full(n, -0.01)]).astype(np.float32)
+    dg = np.column_stack([0.2 * np.cos(phi), 0.2 * np.sin(phi), np.full(n, -0.06)]).astype(np.float32)
+    for sign, expect_hits in ((1.0, False), (-1.0, True)):
+        r = W.pack_rays(og, sign * dg)
+        r[6] = -4.0
+        g = obj.intersect(torch.from_numpy(r).cuda())
+        ref = sc.intersect(og.astype(np.float64), sign * dg.astype(np.float64), np.full(n, -4.0))
+        hit_g, hit_r = torch.isfinite(g["t"]).cpu().numpy(), np.isfinite(ref["t"])
+        if expect_hits:
+            assert hit_r.sum() > 0.9 * n
+            assert (g["t"].cpu().numpy()[hit_g] >= 0).all()
+            assert (hit_g == hit_r).sum() >= n - 2
+        else:
+            assert not hit_r.any() and not hit_g.any()

@@ -148,3 +148,63 @@ def test_full_steps_of_tiny_requests(small_scenes):
     mix = _run(srv, jobs)
     for q, (obj, rays) in enumerate(jobs):
         assert _same(mix[q], _host(obj.intersect(rays))), f"request {q} differs"
+
+
+def test_release_right_after_copy_keeps_the_copy(small_scenes):
+    """dmap_server_copy_hits, then dmap_server_release, then a new request
+    that can only use the released pool region, all before the copy's
+    stream has run: the copy still reads the first request's hits, because
+    the region returns to the pool only when the copy has completed."""
+    import dmap_amd
+    import torch
+    scn = small_scenes[0]
+    n = 8192
+    rays_a = scn.primary[:, :n].contiguous()
+    rays_b = scn.bounce[:, :n].contiguous()
+    plain_a, plain_b = _host(scn.obj.intersect(rays_a)), _host(scn.obj.intersect(rays_b))
+    assert not _same(plain_a, plain_b)
+    # The pool holds one request of n rays (44 B per ray, in 256 B units).
+    pool = ((n * 44 + 255) // 256) * 256 + 256
+    srv = dmap_amd.Server(slots=1, step_rays=1 << 20, hit_pool_bytes=pool)
+    srv.submit(scn.obj, rays_a, 0)
+    while not srv.poll():
+        pass
+    side = torch.cuda.Stream()
+    x = torch.randn((4096, 4096), device="cuda")
+    with torch.cuda.stream(side):
+        y = x
+        for _ in range(12):  # keeps `side` busy for several milliseconds
+            y = (y @ x) * 1e-3
+    got_a = srv.hits(0, n, stream=side)
+    srv.release(0)
+    srv.submit(scn.obj, rays_b, 1)
+    while not srv.poll():
+        pass
+    got_b = _host(srv.hits(1, n))
+    side.synchronize()
+    got_a = {k: v.cpu().numpy() for k, v in got_a.items()}
+    assert _same(got_a, plain_a)
+    assert _same(got_b, plain_b)
+    srv.release(1)
+    torch.cuda.synchronize()
+    assert srv.idle()
+
+
+def test_pool_not_a_multiple_of_256(small_scenes):
+    """A hit pool of any size admits every request it accepts: sizes are
+    compared in the arena's 256 B units, so no accepted request waits for
+    room that can never exist."""
+    import dmap_amd
+    scn = small_scenes[0]
+    n = 1000  # 44,000 B, 44,032 B in 256 B units
+    # 44,020 B holds 44,000 B but only 171 whole units (43,776 B): refused at once.
+    srv = dmap_amd.Server(slots=1, step_rays=1 << 20, hit_pool_bytes=44020)
+    with pytest.raises(dmap_amd.DmapError, match="whole hit pool"):
+        srv.submit(scn.obj, scn.primary[:, :n].contiguous(), 0)
+    srv2 = dmap_amd.Server(slots=1, step_rays=1 << 20, hit_pool_bytes=44032 + 100)
+    rays = scn.primary[:, :n].contiguous()
+    srv2.submit(scn.obj, rays, 0)
+    while not srv2.poll():
+        pass
+    assert _same(_host(srv2.hits(0, n)), _host(scn.obj.intersect(rays)))
+    srv2.release(0)
