@@ -39,7 +39,7 @@ EXPORTS = ("dmap_last_error", "dmap_version", "dmap_ctx_create", "dmap_ctx_destr
            "dmap_device_free", "dmap_memcpy", "dmap_stream_synchronize", "dmap_tile_create",
            "dmap_tile_rebuild", "dmap_tile_info", "dmap_tile_read_level", "dmap_tile_read_nodes",
            "dmap_tile_destroy", "dmap_object_create", "dmap_object_triangle_info", "dmap_object_destroy",
-           "dmap_intersect_closest", "dmap_intersect_any", "dmap_tile_set_texels", "dmap_object_refresh",
+           "dmap_intersect_closest", "dmap_intersect_closest_noalloc", "dmap_intersect_any", "dmap_tile_set_texels", "dmap_object_refresh",
            "dmap_intersect_adjoint", "dmap_hit_record", "dmap_kernel_timing", "dmap_kernel_timing_read", "dmap_server_create", "dmap_server_submit", "dmap_server_poll",
            "dmap_server_hits", "dmap_server_copy_hits", "dmap_server_release", "dmap_server_idle",
            "dmap_server_destroy", "dmap_pathtrace")
@@ -108,6 +108,7 @@ def load_library():
         L.dmap_object_destroy.argtypes = [vp]
         L.dmap_intersect_closest.argtypes = [vp, c.POINTER(_Rays), c.c_int64, c.POINTER(_Hits), c.c_int,
                                              c.POINTER(_Options), c.POINTER(_Stats), vp]
+        L.dmap_intersect_closest_noalloc.argtypes = L.dmap_intersect_closest.argtypes
         L.dmap_intersect_any.argtypes = [vp, c.POINTER(_Rays), c.c_int64, vp, c.c_int, c.POINTER(_Options), vp]
         L.dmap_tile_set_texels.argtypes = [vp, vp, c.c_int, vp]
         L.dmap_kernel_timing.argtypes = [c.c_int]

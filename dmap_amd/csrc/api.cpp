@@ -443,6 +443,17 @@ extern "C" {
 const char *dmap_last_error(void) { return g_error.c_str(); }
 const char *dmap_version(void) { return "dmap_amd 0.1 gfx950"; }
 
+// Scratch of dmap_intersect_closest_noalloc: the ray-queue head (256 B) and
+// the ray reorder buffers (sort keys, permutation, radix sort temporary
+// storage; about 24 B per ray) of one call of at most NOALLOC_MAX_RAYS rays.
+// Every device the library loads on holds one copy for the life of the
+// process. Host code must not use `scratch` itself as an address: its
+// device address comes from hipGetSymbolAddress, per device, in
+// dmap_ctx_create, and is kept in dmap_ctx::scratch.
+constexpr int64_t NOALLOC_MAX_RAYS = (int64_t)1 << 20;
+constexpr size_t NOALLOC_SCRATCH_BYTES = (size_t)32 << 20;
+static __device__ __align__(256) unsigned char scratch[NOALLOC_SCRATCH_BYTES];
+
 int dmap_ctx_create(int device, dmap_ctx **out)
 {
     if (!out)
@@ -452,8 +463,13 @@ int dmap_ctx_create(int device, dmap_ctx **out)
     if (device < 0 || device >= count)
         return fail(DMAP_ERR_INVALID, "dmap_ctx_create: device %d of %d", device, count);
     HIP_TRY(hipSetDevice(device));
+    void *sp = nullptr;
+    hipError_t e = hipGetSymbolAddress(&sp, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !sp)
+        return fail(DMAP_ERR_HIP, "dmap_ctx_create: scratch address on device %d: %s", device, hipGetErrorString(e));
     dmap_ctx *c = new dmap_ctx;
     c->device = device;
+    c->scratch = sp;
     *out = c;
     return DMAP_OK;
 }
@@ -903,8 +919,13 @@ static std::mutex g_timing_mu;
 static bool g_timing = false;
 static std::vector<std::pair<hipEvent_t, hipEvent_t>> g_timing_events;
 
+// fixed_ws: NULL to allocate the call's workspace on the stream; else the
+// device address of a caller-provided workspace of fixed_bytes (the
+// no-allocation entry point's scratch), used instead, with no allocation,
+// free or timing event.
 static int trace(const dmap_object *obj, const dmap_rays *rays, int64_t n, const dmap_hits *hits, uint8_t *occluded,
-                 int prim_mask, const dmap_options *options, const dmap_ray_stats *stats, void *stream)
+                 int prim_mask, const dmap_options *options, const dmap_ray_stats *stats, void *stream,
+                 void *fixed_ws = nullptr, size_t fixed_bytes = 0)
 {
     if (!obj || !rays)
         return fail(DMAP_ERR_INVALID, "intersect: NULL object or rays");
@@ -945,26 +966,33 @@ static int trace(const dmap_object *obj, const dmap_rays *rays, int64_t n, const
     // reorder buffers when asked for.
     bool reorder = o.reorder && n >= 4096;
     size_t rbytes = reorder ? reorder_workspace_bytes(n) : 0;
-    void *ws = nullptr;
-    HIP_TRY(hipMallocAsync(&ws, 256 + rbytes, s));
+    void *ws = fixed_ws;
+    if (fixed_ws) {
+        if (256 + rbytes > fixed_bytes)
+            return fail(DMAP_ERR_INVALID, "intersect: %lld rays need %zu B of scratch, more than its %zu B",
+                        (long long)n, 256 + rbytes, fixed_bytes);
+    } else {
+        HIP_TRY(hipMallocAsync(&ws, 256 + rbytes, s));
+    }
     p.queue = (unsigned *)ws;
     if (reorder) {
         uint32_t *perm = nullptr;
         hipError_t e = reorder_rays(p.rays, n, obj->bound_lo, obj->bound_hi, (char *)ws + 256, rbytes, &perm, s);
         if (e != hipSuccess) {
-            (void)hipFreeAsync(ws, s);
+            if (!fixed_ws)
+                (void)hipFreeAsync(ws, s);
             return fail(DMAP_ERR_HIP, "intersect: ray reorder: %s", hipGetErrorString(e));
         }
         p.perm = perm;
     }
     hipEvent_t ev[2] = {nullptr, nullptr};
-    {
+    if (!fixed_ws) {
         std::lock_guard<std::mutex> lk(g_timing_mu);
         if (g_timing && hipEventCreate(&ev[0]) == hipSuccess && hipEventCreate(&ev[1]) == hipSuccess)
             g_timing_events.emplace_back(ev[0], ev[1]);
     }
     hipError_t e = launch_trace(p, o.precision, s, ev[1] ? ev[0] : nullptr, ev[1]);
-    if (ws)
+    if (!fixed_ws)
         (void)hipFreeAsync(ws, s);
     HIP_TRY(e);
     return DMAP_OK;
@@ -1012,6 +1040,24 @@ int dmap_intersect_closest(const dmap_object *obj, const dmap_rays *rays, int64_
     if (!hits || !hits->t || !hits->s || !hits->tt || !hits->prim || !hits->texel_i || !hits->texel_j)
         return fail(DMAP_ERR_INVALID, "dmap_intersect_closest: hits t, s, tt, prim, texel_i and texel_j are required");
     return trace(obj, rays, n, hits, nullptr, prim_mask, options, stats, stream);
+}
+
+int dmap_intersect_closest_noalloc(const dmap_object *obj, const dmap_rays *rays, int64_t n, const dmap_hits *hits,
+                                   int prim_mask, const dmap_options *options, const dmap_ray_stats *stats,
+                                   void *stream)
+{
+    if (n == 0 && obj && rays)
+        return DMAP_OK;
+    if (!hits || !hits->t || !hits->s || !hits->tt || !hits->prim || !hits->texel_i || !hits->texel_j)
+        return fail(DMAP_ERR_INVALID,
+                    "dmap_intersect_closest_noalloc: hits t, s, tt, prim, texel_i and texel_j are required");
+    if (n > NOALLOC_MAX_RAYS)
+        return fail(DMAP_ERR_INVALID, "dmap_intersect_closest_noalloc: %lld rays, at most %lld per call",
+                    (long long)n, (long long)NOALLOC_MAX_RAYS);
+    if (!obj || !obj->ctx->scratch)
+        return fail(DMAP_ERR_INVALID, "dmap_intersect_closest_noalloc: NULL object or context without scratch");
+    return trace(obj, rays, n, hits, nullptr, prim_mask, options, stats, stream, obj->ctx->scratch,
+                 NOALLOC_SCRATCH_BYTES);
 }
 
 int dmap_intersect_adjoint(const dmap_object *obj, const dmap_rays *rays, int64_t n, const float *dl_dt,

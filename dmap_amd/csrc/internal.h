@@ -15,6 +15,9 @@ using namespace dmap_amd;
 struct dmap_ctx
 {
     int device = 0;
+    // The device address of the no-allocation entry point's scratch array
+    // on this context's device (hipGetSymbolAddress at creation).
+    void *scratch = nullptr;
 };
 
 struct dmap_tile

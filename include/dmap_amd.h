@@ -151,6 +151,20 @@ void dmap_object_destroy(dmap_object *obj);
 int dmap_intersect_closest(const dmap_object *obj, const dmap_rays *rays, int64_t n, const dmap_hits *hits,
                            int prim_mask, const dmap_options *options, const dmap_ray_stats *stats, void *stream);
 
+/* dmap_intersect_closest without any allocation (no counterpart in the
+ * reference, which allocates nothing per ray; this is for callers that
+ * cannot allocate device memory: code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc). Same arguments and
+ * results; its workspace (ray-queue head and ray reorder buffers) is a
+ * fixed 32 MiB device array per GPU, held from library load to process
+ * exit, so one call takes at most 1,048,576 rays (larger n returns
+ * DMAP_ERR_INVALID). Calls that overlap on one GPU share that array and
+ * race: serialize them (one stream, or events). Kernel timing
+ * (dmap_kernel_timing) does not record these calls. */
+int dmap_intersect_closest_noalloc(const dmap_object *obj, const dmap_rays *rays, int64_t n, const dmap_hits *hits,
+                                   int prim_mask, const dmap_options *options, const dmap_ray_stats *stats,
+                                   void *stream);
+
 /* Any hit: occluded[r] = 1 when the ray meets the surface in (tmin, tmax). */
 int dmap_intersect_any(const dmap_object *obj, const dmap_rays *rays, int64_t n, uint8_t *occluded, int prim_mask,
                        const dmap_options *options, void *stream);
