@@ -79,6 +79,14 @@ def uniform(torch, path_id, bounce, dim, seed):
 
 # ------------------------------------------------------------------- tracer
 
+def _length(x):
+    """Euclidean length of the columns of a (3, n) tensor as (x0 x0 + x1 x1) + x2 x2,
+    each product rounded on its own: the device kernels' arithmetic (a fused
+    norm reduction may contract the products into FMAs and round
+    differently)."""
+    return (x * x).sum(0, keepdim=True).sqrt()
+
+
 def _cross(torch, a, b):
     return torch.stack([a[1] * b[2] - a[2] * b[1], a[2] * b[0] - a[0] * b[2], a[0] * b[1] - a[1] * b[0]])
 
@@ -95,10 +103,12 @@ def _light_hit(torch, scene, o, d):
     return torch.where(ok, t, torch.full_like(t, math.inf))
 
 
-def render(torch, scene, cam_o, cam_d, spp=4, depth=4, seed=0, strategy="mis", stats=None):
+def render(torch, scene, cam_o, cam_d, spp=4, depth=4, seed=0, strategy="mis", stats=None, trace_log=None):
     """Radiance per pixel, (n_pixels,) float64. cam_o, cam_d: (3, n_pixels)
     float64 camera rays on the intersector's device. strategy 'mis'
-    (NEE + BSDF with the power heuristic) or 'bsdf' (BSDF sampling only)."""
+    (NEE + BSDF with the power heuristic) or 'bsdf' (BSDF sampling only).
+    trace_log: a list that receives, for every intersector call in order,
+    ("closest" or "occluded", the pixel index of each ray of the batch)."""
     dev = cam_o.device
     n_pix = cam_o.shape[1]
     c, lu, lv, ln = (torch.tensor(x, dtype=torch.float64, device=dev)[:, None] for x in scene.light)
@@ -108,7 +118,7 @@ def render(torch, scene, cam_o, cam_d, spp=4, depth=4, seed=0, strategy="mis", s
         pix = torch.arange(n_pix, device=dev, dtype=torch.int64)
         pid = pix * spp + s
         o, d = cam_o.clone(), cam_d.clone()
-        d = d / d.norm(dim=0, keepdim=True)
+        d = d / _length(d)
         beta = torch.ones(n_pix, dtype=torch.float64, device=dev)
         pdf_prev = torch.zeros(n_pix, dtype=torch.float64, device=dev)  # 0: camera ray (no MIS)
         for b in range(depth + 1):
@@ -116,6 +126,8 @@ def render(torch, scene, cam_o, cam_d, spp=4, depth=4, seed=0, strategy="mis", s
                 break
             hit = scene.its.closest(o, d)
             rays_closest += pix.numel()
+            if trace_log is not None:
+                trace_log.append(("closest", pix.cpu()))
             t_surf = hit["t"]
             t_light = _light_hit(torch, scene, o, d)
             on_light = t_light < t_surf
@@ -145,7 +157,7 @@ def render(torch, scene, cam_o, cam_d, spp=4, depth=4, seed=0, strategy="mis", s
             a = torch.where(ng[0].abs() < 0.9, torch.tensor([1.0, 0.0, 0.0], device=dev, dtype=torch.float64)[:, None],
                             torch.tensor([0.0, 1.0, 0.0], device=dev, dtype=torch.float64)[:, None]).expand_as(ng)
             t1 = _cross(torch, a, ng)
-            t1 = t1 / t1.norm(dim=0, keepdim=True)
+            t1 = t1 / _length(t1)
             t2 = _cross(torch, ng, t1)
             f = scene.albedo / math.pi
             origin = p + SPAWN_EPS * ng
@@ -165,6 +177,8 @@ def render(torch, scene, cam_o, cam_d, spp=4, depth=4, seed=0, strategy="mis", s
                     k = ok.nonzero().squeeze(1)
                     occ = scene.its.occluded(origin[:, k], wi[:, k], dist[k] - LIGHT_EPS)
                     rays_shadow += k.numel()
+                    if trace_log is not None:
+                        trace_log.append(("occluded", pix[k].cpu()))
                     vis = torch.zeros_like(ok)
                     vis[k] = ~occ
                     pdf_l = dist2 / (scene.area * cos_l.clamp_min(1e-12))

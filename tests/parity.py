@@ -85,4 +85,7 @@ def compare(gpu, ref, d, tri_rows, n_leaf, rel=REL):
     okn = ok & ~on_crease
     out["worst_ng"] = float(dng[okn].max()) if okn.any() else 0.0
     out["bad_index"] = np.nonzero((flip | bad_value) & ~band)[0]
+    # Per ray: 0 agrees within the bar, 1 differs inside a band, 2 differs unexplained.
+    differs = flip | bad_value
+    out["ray_flag"] = np.where(differs & ~band, 2, np.where(differs, 1, 0)).astype(np.int8)
     return out
