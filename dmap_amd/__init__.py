@@ -32,7 +32,9 @@ LEAF = {"newton": 0, "two_triangle": 1}
 PRECISION = {"fp32": 0, "fp64": 1}
 BUILD = {"fold": 0, "direct": 1}
 CHANNELS = ("h0", "gu", "gv", "r", "ru", "rv", "h_min", "h_max")
-STAT_NAMES = ("triangle_tests", "node_tests", "leaf_tests", "sub_squares", "newton_runs", "newton_iters")
+STAT_NAMES = ("triangle_tests", "node_tests", "leaf_tests", "sub_squares", "newton_runs", "newton_iters",
+              "node_tests_after_hit", "leaf_tests_after_hit")
+STAT_LEVELS = 16  # DMAP_STAT_LEVELS: node tests per level, levels 0 .. 15
 
 # Exported symbols of include/dmap_amd.h (checked by the CPU tests).
 EXPORTS = ("dmap_last_error", "dmap_version", "dmap_ctx_create", "dmap_ctx_destroy", "dmap_device_alloc",
@@ -71,7 +73,7 @@ class _UvMap(ctypes.Structure):
 
 
 class _Stats(ctypes.Structure):
-    _fields_ = [(n, _FP) for n in STAT_NAMES]
+    _fields_ = [(n, _FP) for n in STAT_NAMES] + [("node_tests_per_level", _FP)]
 
 
 class _PtScene(ctypes.Structure):
@@ -302,8 +304,8 @@ class ObjectIntersector:
         st = None
         stats_t = None
         if stats:
-            stats_t = torch.zeros((len(STAT_NAMES), n), dtype=torch.int32, device="cuda")
-            st = _Stats(*[stats_t[q].data_ptr() for q in range(len(STAT_NAMES))])
+            stats_t = torch.zeros((len(STAT_NAMES) + STAT_LEVELS, n), dtype=torch.int32, device="cuda")
+            st = _Stats(*[stats_t[q].data_ptr() for q in range(len(STAT_NAMES) + 1)])
         opt = (options or TraversalOptions())._c()
         _check(_lib.dmap_intersect_closest(self._h, ctypes.byref(r), n, ctypes.byref(h), int(prim_mask),
                                            ctypes.byref(opt), ctypes.byref(st) if st is not None else None,
@@ -311,6 +313,7 @@ class ObjectIntersector:
         res = {"t": F[0], "st": F[1:3], "bary": F[3:5], "ng": F[5:8], "prim": I[0], "texel": I[1:3]}
         if stats:
             res["stats"] = {k: stats_t[q] for q, k in enumerate(STAT_NAMES)}
+            res["stats"]["node_tests_per_level"] = stats_t[len(STAT_NAMES):]
         return res
 
     def pathtrace(self, cam_o, cam_d, spp, depth, light, radiance, env, albedo, seed=0, strategy="mis",

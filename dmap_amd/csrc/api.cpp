@@ -957,8 +957,9 @@ static int trace(const dmap_object *obj, const dmap_rays *rays, int64_t n, const
                          hits->ny, hits->nz, hits->prim, hits->texel_i, hits->texel_j};
     p.occluded = occluded;
     if (stats)
-        p.stats = StatsDev{stats->triangle_tests, stats->node_tests, stats->leaf_tests, stats->sub_squares,
-                           stats->newton_runs, stats->newton_iters};
+        p.stats = StatsDev{stats->triangle_tests,       stats->node_tests,          stats->leaf_tests,
+                           stats->sub_squares,          stats->newton_runs,         stats->newton_iters,
+                           stats->node_tests_after_hit, stats->leaf_tests_after_hit, stats->node_tests_per_level};
     hipStream_t s = (hipStream_t)stream;
     if (n > 0x7fffffffLL)
         return fail(DMAP_ERR_INVALID, "intersect: at most 2^31 - 1 rays per call");

@@ -43,7 +43,10 @@ struct HitsDev
 struct StatsDev
 {
     int32_t *triangle_tests, *node_tests, *leaf_tests, *sub_squares, *newton_runs, *newton_iters;
+    int32_t *node_tests_after_hit, *leaf_tests_after_hit;
+    int32_t *node_tests_per_level; // STAT_LEVELS planes of n_rays, level-major
 };
+constexpr int STAT_LEVELS = 16; // levels 0 .. MAX_ROOT_LEVEL
 
 // The object's top-level BVH over the triangles' boxes: a complete binary
 // tree in heap order with P = 2^k leaves. Node 1 is the root, node i has

@@ -864,6 +864,20 @@ template <> struct StackSlot<double>
 struct LaneStats
 {
     int tri, node, leaf, sub, runs, iters;
+    // Tests after the last hit, summed over the walks that hit; and those
+    // since the last hit of the current triangle's walk, which hit if found.
+    int node_after, leaf_after, node_since, leaf_since;
+    bool found;
+    // Ends the current triangle's walk (displaced_intersector.cpp:232-235).
+    __device__ void end_walk()
+    {
+        if (found) {
+            node_after += node_since;
+            leaf_after += leaf_since;
+        }
+        node_since = leaf_since = 0;
+        found = false;
+    }
 };
 
 // search_leaf (displaced_intersector.cpp:292-323) as a depth-first search
@@ -1316,7 +1330,7 @@ k_trace(TraceParams p)
     int k = -1;
     int hit_tri = -1, hit_i = 0, hit_j = 0;
     R hit_a = 0, hit_b = 0;
-    LaneStats st{0, 0, 0, 0, 0, 0};
+    LaneStats st{};
     Walk<R> wk;
     const TriRecord *T = p.obj.tris;
     int root_level = 0, rb_i = 0, rb_j = 0;
@@ -1347,6 +1361,11 @@ k_trace(TraceParams p)
             }
             if (STATS) {
                 const StatsDev &S = p.stats;
+                st.end_walk();
+                if (S.node_tests_after_hit)
+                    S.node_tests_after_hit[rid] = st.node_after;
+                if (S.leaf_tests_after_hit)
+                    S.leaf_tests_after_hit[rid] = st.leaf_after;
                 if (S.triangle_tests)
                     S.triangle_tests[rid] = st.tri;
                 if (S.node_tests)
@@ -1419,7 +1438,10 @@ k_trace(TraceParams p)
                         hit_tri = -1;
                         hit_i = hit_j = 0;
                         hit_a = hit_b = 0;
-                        st = LaneStats{0, 0, 0, 0, 0, 0};
+                        st = LaneStats{};
+                        if (STATS && p.stats.node_tests_per_level)
+                            for (int l = 0; l < STAT_LEVELS; ++l)
+                                p.stats.node_tests_per_level[(int64_t)l * p.n_rays + rid] = 0;
                     }
                 }
             }
@@ -1486,8 +1508,10 @@ k_trace(TraceParams p)
                 state = ST_DONE;
                 break;
             }
-            if (STATS)
+            if (STATS) {
+                st.end_walk();
                 ++st.tri;
+            }
             T = DMAP_OBJ.tris + k;
             // The ray in tangent space (DisplacedIntersector::intersect :178),
             // formed in double.
@@ -1528,8 +1552,12 @@ k_trace(TraceParams p)
                                           t_min, t_best, tests);
             if (inside && in_lvl < 0)
                 in_lvl = ex_level;
-            if (STATS)
+            if (STATS) {
                 st.node += tests;
+                st.node_since += tests;
+                if (p.stats.node_tests_per_level)
+                    p.stats.node_tests_per_level[(int64_t)ex_level * p.n_rays + rid] += tests;
+            }
             if (!ex_aligned) {
                 // the root block
                 lvl = root_level;
@@ -1548,8 +1576,10 @@ k_trace(TraceParams p)
         if (leaf_lanes && (leaf_lanes == busy_lanes ||
                            64 * __popcll(leaf_lanes) >= (MULTI ? MULTI_LEAF_BATCH : LEAF_BATCH) * __popcll(busy_lanes))) {
             if (state == ST_LEAF) {
-                if (STATS)
+                if (STATS) {
                     ++st.leaf;
+                    ++st.leaf_since;
+                }
                 const double o[3] = {(double)o_f[0], (double)o_f[1], (double)o_f[2]};
                 const double d[3] = {(double)d_f[0], (double)d_f[1], (double)d_f[2]};
                 Leaf<R> L;
@@ -1566,6 +1596,10 @@ k_trace(TraceParams p)
                     hit_j = leaf_j;
                     hit_a = la;
                     hit_b = lb;
+                    if (STATS) {
+                        st.node_since = st.leaf_since = 0;
+                        st.found = true;
+                    }
                 }
                 state = (found && p.any_hit) ? ST_DONE : ST_NEED;
             }
@@ -2225,7 +2259,8 @@ hipError_t launch_trace(const TraceParams &p, int precision, hipStream_t stream,
     const int64_t want = (p.n_rays + BLOCK - 1) / BLOCK;
     hipLaunchKernelGGL(k_queue_reset, dim3(1), dim3(1), 0, stream, p.queue);
     bool stats = p.stats.node_tests || p.stats.leaf_tests || p.stats.sub_squares || p.stats.triangle_tests ||
-                 p.stats.newton_runs || p.stats.newton_iters;
+                 p.stats.newton_runs || p.stats.newton_iters || p.stats.node_tests_after_hit ||
+                 p.stats.leaf_tests_after_hit || p.stats.node_tests_per_level;
     if (ev_start)
         (void)hipEventRecord(ev_start, stream);
     if (precision == 1) {

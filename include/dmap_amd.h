@@ -82,10 +82,17 @@ typedef struct
 } dmap_hits;
 
 /* Per-ray work counters, TraversalStats (displaced_intersector.h:89-105);
- * device pointers, any may be NULL. */
+ * device pointers, any may be NULL. node_tests_after_hit and
+ * leaf_tests_after_hit count, in each triangle's walk that found a hit, the
+ * tests made after its last hit (displaced_intersector.cpp:181, 222-235).
+ * node_tests_per_level holds DMAP_STAT_LEVELS planes of n counts,
+ * level-major ([level * n + ray]); a triangle's root level is at most 15,
+ * so no node lies above the last plane. */
+#define DMAP_STAT_LEVELS 16
 typedef struct
 {
     int32_t *triangle_tests, *node_tests, *leaf_tests, *sub_squares, *newton_runs, *newton_iters;
+    int32_t *node_tests_after_hit, *leaf_tests_after_hit, *node_tests_per_level;
 } dmap_ray_stats;
 
 const char *dmap_last_error(void);

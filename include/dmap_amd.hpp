@@ -69,7 +69,10 @@ struct DisplacedHit
 
 struct TraversalStats
 {
+    static constexpr int max_levels = DMAP_STAT_LEVELS;
     int64_t triangle_tests = 0, node_tests = 0, leaf_tests = 0, sub_squares = 0;
+    int64_t node_tests_per_level[max_levels] = {};
+    int64_t node_tests_after_hit = 0, leaf_tests_after_hit = 0;
 };
 
 struct RayQuery
@@ -284,11 +287,13 @@ class ObjectIntersector
         dmap_hits hc = h.c();
         dmap_options o = options.c();
         const size_t n = rays.size();
-        DeviceArray<int32_t> counters(*ctx_, stats ? 4 * n : 0);
+        constexpr int planes = 6 + DMAP_STAT_LEVELS;
+        DeviceArray<int32_t> counters(*ctx_, stats ? planes * n : 0);
         dmap_ray_stats st{};
         if (stats) {
             int32_t *c = counters.data();
-            st = dmap_ray_stats{c, c + n, c + 2 * n, c + 3 * n, nullptr, nullptr};
+            st = dmap_ray_stats{c,       c + n,     c + 2 * n, c + 3 * n, nullptr,
+                                nullptr, c + 4 * n, c + 5 * n, c + 6 * n};
         }
         check(dmap_intersect_closest(h_, &rc, (int64_t)n, &hc, prim_mask, &o, stats ? &st : nullptr, nullptr),
               "dmap_intersect_closest");
@@ -301,6 +306,10 @@ class ObjectIntersector
                 out[q].stats.node_tests = c[n + q];
                 out[q].stats.leaf_tests = c[2 * n + q];
                 out[q].stats.sub_squares = c[3 * n + q];
+                out[q].stats.node_tests_after_hit = c[4 * n + q];
+                out[q].stats.leaf_tests_after_hit = c[5 * n + q];
+                for (int l = 0; l < DMAP_STAT_LEVELS; ++l)
+                    out[q].stats.node_tests_per_level[l] = c[(6 + l) * n + q];
             }
         }
         return out;

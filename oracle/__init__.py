@@ -36,7 +36,10 @@ _HERE = Path(__file__).resolve().parent
 LIB_PATH = _HERE / "_build" / "libdmap_oracle.so"
 
 CHANNELS = ("h0", "gu", "gv", "r", "ru", "rv", "h_min", "h_max")
-STAT_NAMES = ("triangle_tests", "node_tests", "leaf_tests", "sub_squares", "newton_runs", "newton_iters")
+STAT_NAMES = ("triangle_tests", "node_tests", "leaf_tests", "sub_squares", "newton_runs", "newton_iters",
+              "node_tests_after_hit", "leaf_tests_after_hit")
+MAX_LEVELS = 32  # TraversalStats::max_levels (displaced_intersector.h:91)
+N_STATS = len(STAT_NAMES) + MAX_LEVELS
 BOUND = {"box": 0, "slab": 1, "both": 2}
 LEAF = {"newton": 0, "two_triangle": 1}
 
@@ -199,13 +202,14 @@ class Scene:
         tmax = np.full(n, np.inf) if tmax is None else _f64(np.broadcast_to(tmax, (n,)))
         out = {"t": np.empty(n), "st": np.empty((n, 2)), "bary": np.empty((n, 2)), "ng": np.empty((n, 3)),
                "prim": np.empty(n, dtype=np.int32), "texel": np.empty((n, 2), dtype=np.int64)}
-        st = np.empty((n, len(STAT_NAMES)), dtype=np.int64) if stats else None
+        st = np.empty((n, N_STATS), dtype=np.int64) if stats else None
         lib().orc_scene_intersect(self.handle, n, _p(o), _p(d), _p(tmin), _p(tmax), BOUND[bound],
                                   int(slab_max_level), LEAF[leaf], int(threads), _p(out["t"]), _p(out["st"]),
                                   _p(out["bary"]), _p(out["ng"]), _p(out["prim"], _I32), _p(out["texel"], _I64),
                                   _p(st, _I64))
         if stats:
             out["stats"] = {k: st[:, q] for q, k in enumerate(STAT_NAMES)}
+            out["stats"]["node_tests_per_level"] = st[:, len(STAT_NAMES):]
         return out
 
     def occluded(self, o, d, tmin=None, tmax=None, bound="both", slab_max_level=1, leaf="newton", threads=0):
@@ -254,10 +258,12 @@ class Scene:
 
     def triangle_intersect(self, k, o, d, tmin=0.0, tmax=np.inf, bound="both", slab_max_level=1, leaf="newton"):
         h = np.zeros(10)
-        st = np.zeros(len(STAT_NAMES), dtype=np.int64)
+        st = np.zeros(N_STATS, dtype=np.int64)
         ok = lib().orc_triangle_intersect(self.handle, k, _p(_f64(o)), _p(_f64(d)), tmin, tmax, BOUND[bound],
                                           slab_max_level, LEAF[leaf], _p(h), _p(st, _I64))
-        return self._hit(ok, h), dict(zip(STAT_NAMES, st.tolist()))
+        out = dict(zip(STAT_NAMES, st.tolist()))
+        out["node_tests_per_level"] = st[len(STAT_NAMES):]
+        return self._hit(ok, h), out
 
     def leaf_newton(self, k, o, d, i, j, tmin=0.0, tmax=np.inf):
         h = np.zeros(10)

@@ -1161,6 +1161,7 @@ struct Intersector
         TRay ray{frame.point(o), frame.direction(d), t_min, t_max};
         bool si = ray.d.x < 0.0, sj = ray.d.y < 0.0;
         bool found = false;
+        int64_t nodes_after = 0, leaves_after = 0; // since the latest hit (displaced_intersector.cpp:181)
         for (int r = 0; r < n_roots; ++r) {
             Cell cur = roots[r], end = roots[r];
             cell_next(end, si, sj, root_level);
@@ -1171,8 +1172,11 @@ struct Intersector
                     cell_next(cur, si, sj, root_level);
                     continue;
                 }
-                if (st)
+                if (st) {
                     ++st->v[ORC_ST_NODE];
+                    ++st->v[ORC_ST_LEVEL0 + std::min(cur.level, (int)ORC_MAX_LEVELS - 1)];
+                    ++nodes_after;
+                }
                 NodeBound nb = node_bound(*tri, frame, *pyr, cur.level, cur.i, cur.j);
                 ray.t_max = t_max;
                 double t0, t1;
@@ -1184,8 +1188,10 @@ struct Intersector
                     cell_down(cur, si, sj);
                     continue;
                 }
-                if (st)
+                if (st) {
                     ++st->v[ORC_ST_LEAF];
+                    ++leaves_after;
+                }
                 Hit lh;
                 bool leaf = opt.leaf == 0 ? intersect_newton(o, d, t_min, t_max, cur.i, cur.j, lh, st)
                                           : intersect_two_triangle(o, d, t_min, t_max, cur.i, cur.j, lh);
@@ -1193,9 +1199,16 @@ struct Intersector
                     hit = lh;
                     t_max = hit.t;
                     found = true;
+                    nodes_after = 0;
+                    leaves_after = 0;
                 }
                 cell_next(cur, si, sj, root_level);
             }
+        }
+        // displaced_intersector.cpp:232-235.
+        if (st && found) {
+            st->v[ORC_ST_NODE_AFTER_HIT] += nodes_after;
+            st->v[ORC_ST_LEAF_AFTER_HIT] += leaves_after;
         }
         return found;
     }

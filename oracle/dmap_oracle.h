@@ -59,7 +59,13 @@ int orc_triangle_info(void *scene, int k, int *root_level, int *n_roots, int64_t
  * except t): t (inf on a miss), st n*2, bary n*2, ng n*3, prim n (-1 on a
  * miss), texel n*2, stats n*ORC_STATS. bound: 0 box, 1 slab, 2 both;
  * leaf: 0 newton, 1 two-triangle. */
-enum { ORC_ST_TRIANGLE = 0, ORC_ST_NODE, ORC_ST_LEAF, ORC_ST_SUB, ORC_ST_NEWTON_RUNS, ORC_ST_NEWTON_ITERS, ORC_STATS };
+/* Per-ray stats (TraversalStats, displaced_intersector.h:89-105): the six
+ * counts, the node and leaf tests made after the walk's final hit in walks
+ * that hit, then node tests per level (levels >= ORC_MAX_LEVELS - 1 in the
+ * last entry). */
+enum { ORC_ST_TRIANGLE = 0, ORC_ST_NODE, ORC_ST_LEAF, ORC_ST_SUB, ORC_ST_NEWTON_RUNS, ORC_ST_NEWTON_ITERS,
+       ORC_ST_NODE_AFTER_HIT, ORC_ST_LEAF_AFTER_HIT, ORC_ST_LEVEL0 };
+enum { ORC_MAX_LEVELS = 32, ORC_STATS = ORC_ST_LEVEL0 + ORC_MAX_LEVELS };
 void orc_scene_intersect(void *scene, int64_t n, const double *o, const double *d, const double *tmin,
                          const double *tmax, int bound, int slab_max_level, int leaf, int nthreads, double *t,
                          double *st, double *bary, double *ng, int32_t *prim, int64_t *texel, int64_t *stats);

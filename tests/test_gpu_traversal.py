@@ -32,7 +32,7 @@ def run_both(wl, idx=None, options=None, oracle_kw=None):
 
 
 def report(cmp):
-    return {k: v for k, v in cmp.items() if k != "bad_index"}
+    return {k: v for k, v in cmp.items() if k not in ("bad_index", "ray_flag")}
 
 
 def test_c1_closest_hit_parity(torch_gpu):
@@ -44,6 +44,31 @@ def test_c1_closest_hit_parity(torch_gpu):
     assert cmp["unexplained"] == 0, report(cmp)
     # Work counters: the padded fp32 tests keep every node the double tests keep.
     assert g["stats"]["node_tests"].sum() >= ref["stats"]["node_tests"].sum()
+    check_counters(g, ref, "C1")
+
+
+def check_counters(g, ref, label):
+    """The per-ray counters of TraversalStats (displaced_intersector.h:89-105)
+    on the GPU: node tests per level sum to the node tests; tests after the
+    final hit are 0 on a miss, within the totals, and exclude the hit's leaf.
+    Against the oracle, level by level over the rays: the GPU's padded fp32
+    tests keep every node the double tests keep and test a passing cell's
+    four children together, so it tests at least as many nodes per level."""
+    G, O = g["stats"], ref["stats"]
+    per = G["node_tests_per_level"]
+    assert per.shape[0] == 16
+    assert (per.sum(axis=0) == G["node_tests"]).all()
+    hit = np.isfinite(g["t"])
+    for name, total in (("node_tests_after_hit", "node_tests"), ("leaf_tests_after_hit", "leaf_tests")):
+        assert (G[name][~hit] == 0).all() and (G[name] <= G[total]).all()
+    assert (G["leaf_tests_after_hit"][hit] <= G["leaf_tests"][hit] - 1).all()
+    gl = per.sum(axis=1).astype(np.int64)
+    ol = O["node_tests_per_level"].sum(axis=0)
+    assert ol[16:].sum() == 0
+    print(f"[{label}] node tests per level GPU {gl.tolist()} oracle {ol[:16].tolist()}; after hit: nodes "
+          f"{int(G['node_tests_after_hit'].sum())} / {int(O['node_tests_after_hit'].sum())}, leaves "
+          f"{int(G['leaf_tests_after_hit'].sum())} / {int(O['leaf_tests_after_hit'].sum())}", flush=True)
+    assert (gl >= ol[:16]).all()
 
 
 def test_c2_small_closest_hit_parity(torch_gpu):
@@ -53,3 +78,4 @@ def test_c2_small_closest_hit_parity(torch_gpu):
     cmp = compare(g, ref, d, wl.triangles, wl.texels.shape[0])
     print(report(cmp))
     assert cmp["unexplained"] == 0, report(cmp)
+    check_counters(g, ref, "C2 at 256^2")

@@ -287,3 +287,29 @@ def test_root_level_grows_with_the_uv_span():
         assert np.isfinite(res["t"]).all()
         S, _ = sc.surface_points(res["prim"], res["st"])
         assert np.linalg.norm(S - (o + res["t"][:, None] * d), axis=1).max() <= 2 * sc.triangle_info(0)["residual_tolerance"]
+
+
+def test_work_counters_per_level_and_after_hit():
+    """TraversalStats (displaced_intersector.h:89-105): node tests per level
+    sum to the node tests; `both` at every slab level k tests no more nodes
+    than `box` on every ray at every level (test_traversal_cost.cpp:268-297,
+    check (1)); tests after the final hit are counted in walks that hit only,
+    never exceed the totals, and exclude the leaf that made the hit."""
+    sc, _, _ = small_scene(n=64, seed=8)
+    o, d = ray_set(4000, seed=4)
+    box = sc.intersect(o, d, bound="box", stats=True)
+    bs = box["stats"]
+    assert (bs["node_tests_per_level"].sum(axis=1) == bs["node_tests"]).all()
+    assert bs["node_tests_per_level"][:, 7:].sum() == 0  # a 64^2 map has levels 0..6
+    for k in range(0, 7):
+        both = sc.intersect(o, d, bound="both", slab_max_level=k, stats=True)
+        assert (both["stats"]["node_tests_per_level"] <= bs["node_tests_per_level"]).all(), k
+    hit = np.isfinite(box["t"])
+    assert hit.sum() > 1000 and (~hit).sum() > 100
+    for name, total in (("node_tests_after_hit", "node_tests"), ("leaf_tests_after_hit", "leaf_tests")):
+        after = bs[name]
+        assert (after[~hit] == 0).all()
+        assert (after <= bs[total]).all()
+    assert (bs["leaf_tests_after_hit"][hit] <= bs["leaf_tests"][hit] - 1).all()
+    # Some walks go on after their final hit (the child order is not front to back).
+    assert bs["node_tests_after_hit"][hit].sum() > 0

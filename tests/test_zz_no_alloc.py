@@ -37,8 +37,8 @@ def _call(dmap_amd, torch, obj, rays, entry, stats=False, stream=None):
     S = None
     st = None
     if stats:
-        S = torch.zeros((len(dmap_amd.STAT_NAMES), n), dtype=torch.int32, device="cuda")
-        st = dmap_amd._Stats(*[S[q].data_ptr() for q in range(len(dmap_amd.STAT_NAMES))])
+        S = torch.zeros((len(dmap_amd.STAT_NAMES) + dmap_amd.STAT_LEVELS, n), dtype=torch.int32, device="cuda")
+        st = dmap_amd._Stats(*[S[q].data_ptr() for q in range(len(dmap_amd.STAT_NAMES) + 1)])
     opt = dmap_amd.TraversalOptions()._c()
     fn = getattr(dmap_amd._lib, entry)
     rc = fn(obj._h, ctypes.byref(r), n, ctypes.byref(h), -1, ctypes.byref(opt),
