@@ -53,7 +53,7 @@ class DmapError(RuntimeError):
 
 class _Options(ctypes.Structure):
     _fields_ = [("bound", ctypes.c_int), ("slab_max_level", ctypes.c_int), ("leaf", ctypes.c_int),
-                ("precision", ctypes.c_int), ("reorder", ctypes.c_int)]
+                ("precision", ctypes.c_int), ("reorder", ctypes.c_int), ("polish", ctypes.c_int)]
 
 
 _FP = ctypes.c_void_p
@@ -154,10 +154,11 @@ class TraversalOptions:
     leaf: str = "newton"
     precision: str = "fp32"
     reorder: bool = True
+    polish: bool = True  # False: report the search's own root, without the double polish
 
     def _c(self):
         return _Options(BOUND[self.bound], int(self.slab_max_level), LEAF[self.leaf], PRECISION[self.precision],
-                        1 if self.reorder else 0)
+                        1 if self.reorder else 0, 1 if self.polish else 0)
 
 
 class Context:

@@ -904,7 +904,7 @@ void dmap_object_destroy(dmap_object *obj)
 
 static int check_options(const dmap_options *opt, dmap_options &o)
 {
-    o = dmap_options{2, 1, 0, 0, 1};
+    o = dmap_options{2, 1, 0, 0, 1, 1};
     if (opt)
         o = *opt;
     if (o.bound < 0 || o.bound > 2 || o.leaf < 0 || o.leaf > 1 || o.precision < 0 || o.precision > 1)
@@ -949,6 +949,7 @@ static int trace(const dmap_object *obj, const dmap_rays *rays, int64_t n, const
     p.bound = o.bound;
     p.slab_max_level = o.slab_max_level;
     p.leaf = o.leaf;
+    p.polish = o.polish ? 1 : 0;
     p.any_hit = occluded ? 1 : 0;
     p.n_rays = n;
     p.rays = RaysDev{rays->ox, rays->oy, rays->oz, rays->dx, rays->dy, rays->dz, rays->tmin, rays->tmax};

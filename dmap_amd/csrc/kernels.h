@@ -106,6 +106,7 @@ struct TraceParams
     int bound;     // 0 box, 1 slab, 2 both
     int slab_max_level;
     int leaf; // 0 newton, 1 two-triangle
+    int polish; // 1: the reported Newton root is polished in double (two steps)
     int any_hit;
     int64_t n_rays;
     const uint32_t *perm; // lane order of the rays (NULL: as given)

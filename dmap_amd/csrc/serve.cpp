@@ -352,6 +352,7 @@ int dmap_server_create(dmap_ctx *ctx, int n_slots, int64_t step_rays, int64_t hi
         p.bound = 2;
         p.slab_max_level = 1;
         p.leaf = 0;
+        p.polish = 1;
         for (int kind = 0; kind < 2 && e == hipSuccess; ++kind) {
             hipGraph_t g = nullptr;
             e = hipStreamBeginCapture(sl->stream, hipStreamCaptureModeThreadLocal);

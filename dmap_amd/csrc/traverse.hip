@@ -1672,10 +1672,12 @@ __device__ __forceinline__ void finalize_hit(const TraceParams &p, const Pyramid
             Sb[q] = Sc[C][q] - Sc[A][q];
         }
     } else {
-        polish_root(TH, L, a, b);
-        double F1, F2, along, J00, J01, J10, J11;
-        newton_eval<double, false>(TH, L, a, b, F1, F2, along, J00, J01, J10, J11);
-        t_hit = along / L.d_len;
+        if (p.polish) {
+            polish_root(TH, L, a, b);
+            double F1, F2, along, J00, J01, J10, J11;
+            newton_eval<double, false>(TH, L, a, b, F1, F2, along, J00, J01, J10, J11);
+            t_hit = along / L.d_len;
+        }
         double h = ((L.h00 + L.ha * a) + L.hb * b) + L.tw * a * b;
         double hda = L.ha + L.tw * b, hdb = L.hb + L.tw * a;
         if (TH.const_normal) {

@@ -65,6 +65,9 @@ typedef struct
     int precision;      /* 0 fp32 (default), 1 the same solve in fp64 on the device */
     int reorder;        /* 1 (default): sort the rays by direction octant and origin before
                            tracing, for wave coherence; results are per ray either way */
+    int polish;         /* 1 (default): the reported root is polished by two Newton steps in
+                           double; 0 reports the search's own root (precision's arithmetic),
+                           evaluated in double, to measure the fp32 solve's error */
 } dmap_options;
 
 /* Device pointers, n entries each. */

@@ -51,10 +51,11 @@ struct TraversalOptions
     LeafMode leaf = LeafMode::Newton;
     bool fp64 = false;   // the same solve in double on the device
     bool reorder = true; // sort rays for wave coherence (results are per ray either way)
+    bool polish = true;  // polish the reported root in double (false: the search's own root)
 
     dmap_options c() const
     {
-        return dmap_options{(int)bound, slab_max_level, (int)leaf, fp64 ? 1 : 0, reorder ? 1 : 0};
+        return dmap_options{(int)bound, slab_max_level, (int)leaf, fp64 ? 1 : 0, reorder ? 1 : 0, polish ? 1 : 0};
     }
 };
 
