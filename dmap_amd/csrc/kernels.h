@@ -79,17 +79,41 @@ struct ObjectDev
     int const_normal; // every triangle has a constant interpolated normal
 };
 
-// One request's share of a serving step: `count` rays at step slots
-// [start, start + count), which are the request's rays first .. first +
-// count (rays and hits point at the request's arrays).
+// One request's share of a serving step: the step slots from `start` up to
+// the next segment's start hold the request's rays `first`, first + 1, ...
+// The request's ray planes are at `rays` + q * rays_stride (q = ox .. tmax)
+// when they lie at one stride, which is the usual case; with rays_stride 0,
+// `rays` points at a RaysDev record in device memory. Its hits are its
+// region of the server's pool: 8 fp32 planes of hits_n values (t, s, tt,
+// bu, bv, nx, ny, nz), then 3 int32 planes (prim, texel_i, texel_j).
 struct Segment
 {
-    int64_t start, count, first;
+    int64_t start, first;
     const ObjectDev *obj;
-    RaysDev rays;
-    HitsDev hits;
+    const float *rays;
+    int64_t rays_stride;
+    float *hits;
+    int64_t hits_n;
 };
-constexpr int MAX_SEGMENTS = 256;
+static_assert(sizeof(Segment) == 56, "a step segment is 56 bytes");
+__host__ __device__ inline RaysDev segment_rays(const Segment &g)
+{
+    if (g.rays_stride == 0)
+        return *(const RaysDev *)g.rays;
+    const float *r = g.rays;
+    const int64_t k = g.rays_stride;
+    return RaysDev{r, r + k, r + 2 * k, r + 3 * k, r + 4 * k, r + 5 * k, r + 6 * k, r + 7 * k};
+}
+__host__ __device__ inline HitsDev segment_hits(const Segment &g)
+{
+    float *f = g.hits;
+    const int64_t n = g.hits_n;
+    int32_t *i = (int32_t *)(f + 8 * n);
+    return HitsDev{f, f + n, f + 2 * n, f + 3 * n, f + 4 * n, f + 5 * n, f + 6 * n, f + 7 * n, i, i + n, i + 2 * n};
+}
+// A step's descriptor travels in the kernels' arguments (no upload before
+// the launch): at most MAX_SEGMENTS requests per step keep it under 3 KB.
+constexpr int MAX_SEGMENTS = 48;
 constexpr int64_t MAX_STEP_RAYS = (int64_t)1 << 30;
 struct StepDesc
 {
@@ -101,7 +125,6 @@ struct StepDesc
 struct TraceParams
 {
     ObjectDev obj;        // single-object launches
-    const StepDesc *step; // multi-request launches (NULL for single-object ones)
     int prim_mask; // -1: every triangle; k: triangle k only
     int bound;     // 0 box, 1 slab, 2 both
     int slab_max_level;
@@ -158,11 +181,20 @@ hipError_t launch_hit_record(const ObjectDev &obj, const RaysDev &rays, const Hi
 // after the traversal kernel (for its own duration).
 hipError_t launch_trace(const TraceParams &p, int precision, hipStream_t stream, hipEvent_t ev_start = nullptr,
                         hipEvent_t ev_end = nullptr);
-// The multi-request closest hit of serving: reset, trace and finalize over
-// the step descriptor at p.step (fp32, Newton leaf, no counters), with a
-// grid that does not depend on the step, so the three launches can be
-// captured once in a graph and replayed with new descriptors. Every object
-// of the step has constant normals (const_normal) or none has.
-hipError_t launch_trace_multi(const TraceParams &p, bool const_normal, hipStream_t stream);
+// The multi-request closest hit of serving as a HIP graph: reset of the
+// step's ray queue, trace and finalize (fp32, Newton leaf, no counters),
+// with grids that do not depend on the step. Each launch sets the trace and
+// finalize nodes' arguments to the step's descriptor and replays the graph.
+// Every object of a step has constant normals (const_normal) or none has.
+struct StepGraph
+{
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    hipGraphNode_t trace = nullptr, finalize = nullptr;
+    bool const_normal = false;
+};
+hipError_t step_graph_create(unsigned *queue, bool const_normal, StepGraph &g);
+hipError_t step_graph_launch(StepGraph &g, const TraceParams &p, const StepDesc &desc, hipStream_t stream);
+void step_graph_destroy(StepGraph &g);
 
 } // namespace dmap_amd

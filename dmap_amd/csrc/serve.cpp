@@ -19,13 +19,17 @@
 //    interpolated normals use different kernel specialisations, so a step
 //    takes requests of the kind of its first request only.
 // A step is one persistent multi-request launch (reset, trace, finalize)
-// captured once per slot and kind as a HIP graph and replayed with the
-// step's descriptor, so requests that arrive while others run are admitted
-// into the next free slot and overlap on the device with the steps already
-// running. Each step records an event; the server's calls (submit, poll,
-// release, idle) query the events, and a request's latency is the time its
-// last step was seen complete minus its submission, so a caller that polls
-// continuously measures it to within one poll.
+// built once per slot and kind as a HIP graph and replayed with the step's
+// descriptor in the kernels' arguments (no upload kernel before it), so
+// requests that arrive while others run are admitted into the next free
+// slot and overlap on the device with the steps already running. (A second
+// step queued behind the running one on each slot's stream left the
+// saturated rate unchanged and doubled the median latency at a fixed load,
+// so each slot runs one step at a time; STEPS_PER_SLOT keeps the choice
+// open.) Each step records an event; the server's calls
+// (submit, poll, release, idle) query the events, and a request's latency
+// is the time its last step was seen complete minus its submission, so a
+// caller that polls continuously measures it to within one poll.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -105,6 +109,11 @@ struct Request
     uint64_t seq = 0; // arrival order
     const dmap_object *obj = nullptr;
     dmap_rays rays{};
+    // The ray planes as a step segment takes them: base + q * stride, or
+    // (stride 0) a RaysDev record in the server's device table, entry `table`.
+    const float *rays_base = nullptr;
+    int64_t rays_stride = 0;
+    int table = -1;
     int64_t n = 0;
     int64_t next = 0; // rays handed to steps so far
     int inflight = 0; // steps holding a chunk of this request
@@ -124,17 +133,26 @@ struct Deferred
     std::vector<hipEvent_t> copies;
 };
 
-struct Slot
+// One step in flight on a slot's stream.
+struct StepBuf
 {
-    hipStream_t stream = nullptr;
-    StepDesc *desc_host = nullptr; // pinned
-    StepDesc *desc_dev = nullptr;
-    unsigned *queue = nullptr;
-    hipGraphExec_t graph[2] = {nullptr, nullptr}; // by kind: interpolated, constant normals
-    hipEvent_t done = nullptr;                     // recorded after the step's launches
+    unsigned *queue = nullptr; // the step's ray queue head (device)
+    StepGraph graph[2];        // by kind: interpolated, constant normals
+    hipEvent_t done = nullptr; // recorded after the step's launches
     std::vector<Request *> reqs;
     bool busy = false;
 };
+
+// Steps queued on one stream: at most STEPS_PER_SLOT.
+constexpr int STEPS_PER_SLOT = 1;
+struct Slot
+{
+    hipStream_t stream = nullptr;
+    StepBuf step[STEPS_PER_SLOT];
+};
+
+// RaysDev records of requests whose ray planes do not lie at one stride.
+constexpr int RAY_TABLE = 1024;
 
 } // namespace
 
@@ -149,6 +167,8 @@ struct dmap_server
     std::map<uint64_t, Request *> live; // submitted, not released
     std::vector<Request *> finished;    // completed, not yet polled
     std::vector<Deferred> deferred;     // released regions waiting for their copies
+    RaysDev *ray_table = nullptr;       // device, RAY_TABLE records
+    std::vector<int> table_free;
     std::mutex mu;
     uint64_t submitted = 0;
     int64_t completed_rays = 0;
@@ -175,16 +195,6 @@ dmap_hits hits_in(char *base, int64_t n)
     h.texel_i = i + n;
     h.texel_j = i + 2 * n;
     return h;
-}
-
-HitsDev hits_dev(const dmap_hits &h)
-{
-    return HitsDev{h.t, h.s, h.tt, h.bu, h.bv, h.nx, h.ny, h.nz, h.prim, h.texel_i, h.texel_j};
-}
-
-RaysDev rays_dev(const dmap_rays &r)
-{
-    return RaysDev{r.ox, r.oy, r.oz, r.dx, r.dy, r.dz, r.tmin, r.tmax};
 }
 
 // The pool bytes of a request of n rays, as the arena rounds them.
@@ -216,93 +226,117 @@ void reclaim(dmap_server *S)
     }
 }
 
-// Collect finished slots; then fill every free slot with a new step.
+// Collects finished steps; then forms new steps for free step buffers: first
+// on streams with nothing in flight, then (STEPS_PER_SLOT > 1) queued behind
+// a running step.
 int pump(dmap_server *S)
 {
     reclaim(S);
     for (Slot *sl : S->slots) {
-        if (!sl->busy)
-            continue;
-        hipError_t q = hipEventQuery(sl->done);
-        if (q == hipErrorNotReady)
-            continue;
-        if (q != hipSuccess)
-            return fail(DMAP_ERR_HIP, "serving step: %s", hipGetErrorString(q));
-        const double t = now_s();
-        for (Request *r : sl->reqs) {
-            if (--r->inflight == 0 && r->next == r->n) {
-                r->t_done = t;
-                r->done = true;
-                S->finished.push_back(r);
-                S->completed_rays += r->n;
-            }
-        }
-        sl->reqs.clear();
-        sl->busy = false;
-    }
-    const double now = now_s();
-    for (Slot *sl : S->slots) {
-        if (sl->busy || S->pending.empty())
-            continue;
-        std::vector<Request *> order(S->pending);
-        std::sort(order.begin(), order.end(), [now](const Request *a, const Request *b) {
-            bool aa = now - a->t_submit > AGING_S, ab = now - b->t_submit > AGING_S;
-            if (aa != ab)
-                return aa;
-            if (!aa) {
-                int64_t ra = a->n - a->next, rb = b->n - b->next;
-                if (ra != rb)
-                    return ra < rb;
-            }
-            return a->seq < b->seq;
-        });
-        StepDesc &D = *sl->desc_host;
-        const int kind = order.front()->obj->const_normal ? 1 : 0;
-        int64_t total = 0;
-        int n_seg = 0;
-        for (Request *r : order) {
-            if (n_seg == MAX_SEGMENTS || total >= S->step_rays)
-                break;
-            if ((r->obj->const_normal ? 1 : 0) != kind)
+        for (StepBuf &b : sl->step) {
+            if (!b.busy)
                 continue;
-            if (!r->admitted) {
-                size_t bytes = request_bytes(r->n);
-                size_t off;
-                if (!S->arena.alloc(bytes, off)) {
-                    if (now - r->t_submit > AGING_S)
-                        break; // an aged request waits for room first
-                    continue;
+            hipError_t q = hipEventQuery(b.done);
+            if (q == hipErrorNotReady)
+                continue;
+            if (q != hipSuccess)
+                return fail(DMAP_ERR_HIP, "serving step: %s", hipGetErrorString(q));
+            const double t = now_s();
+            for (Request *r : b.reqs) {
+                if (--r->inflight == 0 && r->next == r->n) {
+                    r->t_done = t;
+                    r->done = true;
+                    S->finished.push_back(r);
+                    S->completed_rays += r->n;
                 }
-                r->admitted = true;
-                r->off = off;
-                r->bytes = bytes;
-                r->hits = hits_in(S->arena.base + off, r->n);
             }
-            int64_t take = std::min(r->n - r->next, S->step_rays - total);
-            Segment &g = D.seg[n_seg++];
-            g.start = total;
-            g.count = take;
-            g.first = r->next;
-            g.obj = r->obj->view_dev;
-            g.rays = rays_dev(r->rays);
-            g.hits = hits_dev(r->hits);
-            r->next += take;
-            ++r->inflight;
-            total += take;
-            sl->reqs.push_back(r);
+            b.reqs.clear();
+            b.busy = false;
         }
-        if (n_seg == 0)
-            break; // nothing fits the hit pool yet
-        S->pending.erase(std::remove_if(S->pending.begin(), S->pending.end(),
-                                        [](const Request *r) { return r->next == r->n; }),
-                         S->pending.end());
-        D.total = total;
-        D.n_seg = n_seg;
-        size_t desc_bytes = offsetof(StepDesc, seg) + sizeof(Segment) * n_seg;
-        sl->busy = true;
-        HIP_TRY(hipMemcpyAsync(sl->desc_dev, sl->desc_host, desc_bytes, hipMemcpyHostToDevice, sl->stream));
-        HIP_TRY(hipGraphLaunch(sl->graph[kind], sl->stream));
-        HIP_TRY(hipEventRecord(sl->done, sl->stream));
+    }
+    for (int depth = 0; depth < STEPS_PER_SLOT; ++depth) {
+        for (Slot *sl : S->slots) {
+            if (S->pending.empty())
+                return DMAP_OK;
+            int inflight = 0;
+            StepBuf *b = nullptr;
+            for (StepBuf &c : sl->step) {
+                if (c.busy)
+                    ++inflight;
+                else if (!b)
+                    b = &c;
+            }
+            if (inflight != depth || !b)
+                continue;
+            const double now = now_s();
+            std::vector<Request *> order(S->pending);
+            std::sort(order.begin(), order.end(), [now](const Request *a, const Request *c) {
+                bool aa = now - a->t_submit > AGING_S, ac = now - c->t_submit > AGING_S;
+                if (aa != ac)
+                    return aa;
+                if (!aa) {
+                    int64_t ra = a->n - a->next, rc = c->n - c->next;
+                    if (ra != rc)
+                        return ra < rc;
+                }
+                return a->seq < c->seq;
+            });
+            StepDesc D;
+            D.pad_ = 0;
+            const int kind = order.front()->obj->const_normal ? 1 : 0;
+            int64_t total = 0;
+            int n_seg = 0;
+            for (Request *r : order) {
+                if (n_seg == MAX_SEGMENTS || total >= S->step_rays)
+                    break;
+                if ((r->obj->const_normal ? 1 : 0) != kind)
+                    continue;
+                if (!r->admitted) {
+                    size_t bytes = request_bytes(r->n);
+                    size_t off;
+                    if (!S->arena.alloc(bytes, off)) {
+                        if (now - r->t_submit > AGING_S)
+                            break; // an aged request waits for room first
+                        continue;
+                    }
+                    r->admitted = true;
+                    r->off = off;
+                    r->bytes = bytes;
+                    r->hits = hits_in(S->arena.base + off, r->n);
+                }
+                int64_t take = std::min(r->n - r->next, S->step_rays - total);
+                Segment &g = D.seg[n_seg++];
+                g.start = total;
+                g.first = r->next;
+                g.obj = r->obj->view_dev;
+                g.rays = r->rays_base;
+                g.rays_stride = r->rays_stride;
+                g.hits = r->hits.t;
+                g.hits_n = r->n;
+                r->next += take;
+                ++r->inflight;
+                total += take;
+                b->reqs.push_back(r);
+            }
+            if (n_seg == 0)
+                return DMAP_OK; // nothing fits the hit pool yet
+            S->pending.erase(std::remove_if(S->pending.begin(), S->pending.end(),
+                                            [](const Request *r) { return r->next == r->n; }),
+                             S->pending.end());
+            D.total = total;
+            D.n_seg = n_seg;
+            TraceParams p;
+            std::memset(&p, 0, sizeof p);
+            p.queue = b->queue;
+            p.prim_mask = -1;
+            p.bound = 2;
+            p.slab_max_level = 1;
+            p.leaf = 0;
+            p.polish = 1;
+            b->busy = true;
+            HIP_TRY(step_graph_launch(b->graph[kind], p, D, sl->stream));
+            HIP_TRY(hipEventRecord(b->done, sl->stream));
+        }
     }
     return DMAP_OK;
 }
@@ -328,47 +362,21 @@ int dmap_server_create(dmap_ctx *ctx, int n_slots, int64_t step_rays, int64_t hi
     }
     // The arena hands out whole 256 B units.
     S->arena.init(S->arena_dev, (size_t)hit_pool_bytes & ~(size_t)255);
-    for (int k = 0; k < n_slots; ++k) {
+    e = hipMalloc(&S->ray_table, sizeof(RaysDev) * RAY_TABLE);
+    for (int k = RAY_TABLE - 1; k >= 0; --k)
+        S->table_free.push_back(k);
+    for (int k = 0; k < n_slots && e == hipSuccess; ++k) {
         Slot *sl = new Slot;
         S->slots.push_back(sl);
         e = hipStreamCreateWithFlags(&sl->stream, hipStreamNonBlocking);
-        if (e == hipSuccess)
-            e = hipHostMalloc((void **)&sl->desc_host, sizeof(StepDesc), hipHostMallocDefault);
-        if (e == hipSuccess)
-            e = hipMalloc(&sl->desc_dev, sizeof(StepDesc));
-        if (e == hipSuccess)
-            e = hipMalloc(&sl->queue, 256);
-        if (e == hipSuccess)
-            e = hipEventCreateWithFlags(&sl->done, hipEventDisableTiming);
-        if (e != hipSuccess)
-            break;
-        // Capture the step's three launches once; they read the descriptor
-        // from desc_dev, so a replay after a new copy runs the new step.
-        TraceParams p;
-        std::memset(&p, 0, sizeof p);
-        p.step = sl->desc_dev;
-        p.queue = sl->queue;
-        p.prim_mask = -1;
-        p.bound = 2;
-        p.slab_max_level = 1;
-        p.leaf = 0;
-        p.polish = 1;
-        for (int kind = 0; kind < 2 && e == hipSuccess; ++kind) {
-            hipGraph_t g = nullptr;
-            e = hipStreamBeginCapture(sl->stream, hipStreamCaptureModeThreadLocal);
-            if (e == hipSuccess) {
-                hipError_t le = launch_trace_multi(p, kind == 1, sl->stream);
-                e = hipStreamEndCapture(sl->stream, &g);
-                if (le != hipSuccess)
-                    e = le;
-            }
+        for (StepBuf &b : sl->step) {
             if (e == hipSuccess)
-                e = hipGraphInstantiate(&sl->graph[kind], g, nullptr, nullptr, 0);
-            if (g)
-                (void)hipGraphDestroy(g);
+                e = hipMalloc(&b.queue, 256);
+            if (e == hipSuccess)
+                e = hipEventCreateWithFlags(&b.done, hipEventDisableTiming);
+            for (int kind = 0; kind < 2 && e == hipSuccess; ++kind)
+                e = step_graph_create(b.queue, kind == 1, b.graph[kind]);
         }
-        if (e != hipSuccess)
-            break;
     }
     if (e != hipSuccess) {
         dmap_server_destroy(S);
@@ -388,17 +396,40 @@ int dmap_server_submit(dmap_server *S, const dmap_object *obj, const dmap_rays *
     std::lock_guard<std::mutex> lk(S->mu);
     if (S->live.count(tag))
         return fail(DMAP_ERR_INVALID, "dmap_server_submit: tag %llu is in use", (unsigned long long)tag);
+    const float *planes[8] = {rays->ox, rays->oy, rays->oz, rays->dx, rays->dy, rays->dz, rays->tmin, rays->tmax};
+    for (const float *pl : planes)
+        if (!pl)
+            return fail(DMAP_ERR_INVALID, "dmap_server_submit: a ray array is NULL");
+    HIP_TRY(hipSetDevice(S->ctx->device));
+    // The step segments take the ray planes as base + q * stride; planes
+    // elsewhere go through a RaysDev record in the device table.
+    const int64_t stride = planes[1] - planes[0];
+    bool uniform = stride != 0;
+    for (int q = 2; q < 8 && uniform; ++q)
+        uniform = planes[q] - planes[0] == q * stride;
+    int table = -1;
+    if (!uniform) {
+        if (S->table_free.empty())
+            return fail(DMAP_ERR_INVALID, "dmap_server_submit: more than %d live requests whose ray arrays do not "
+                        "lie at one stride", RAY_TABLE);
+        table = S->table_free.back();
+        RaysDev rec{planes[0], planes[1], planes[2], planes[3], planes[4], planes[5], planes[6], planes[7]};
+        HIP_TRY(hipMemcpy(S->ray_table + table, &rec, sizeof rec, hipMemcpyHostToDevice));
+        S->table_free.pop_back();
+    }
     Request *r = new Request;
     r->tag = tag;
     r->seq = S->submitted;
     r->obj = obj;
     r->rays = *rays;
+    r->rays_base = uniform ? planes[0] : (const float *)(S->ray_table + table);
+    r->rays_stride = uniform ? stride : 0;
+    r->table = table;
     r->n = n;
     r->t_submit = now_s();
     S->live[tag] = r;
     S->pending.push_back(r);
     ++S->submitted;
-    HIP_TRY(hipSetDevice(S->ctx->device));
     return pump(S);
 }
 
@@ -486,6 +517,8 @@ int dmap_server_release(dmap_server *S, uint64_t tag)
             S->deferred.push_back(Deferred{r->off, r->bytes, std::move(r->copies)});
     }
     r->copies.clear();
+    if (r->table >= 0)
+        S->table_free.push_back(r->table);
     S->live.erase(it);
     delete r;
     return pump(S);
@@ -500,7 +533,8 @@ int dmap_server_idle(dmap_server *S, int *idle)
     int rc = pump(S);
     bool any = !S->pending.empty();
     for (Slot *sl : S->slots)
-        any = any || sl->busy;
+        for (const StepBuf &b : sl->step)
+            any = any || b.busy;
     *idle = any ? 0 : 1;
     return rc;
 }
@@ -513,16 +547,15 @@ void dmap_server_destroy(dmap_server *S)
     for (Slot *sl : S->slots) {
         if (sl->stream)
             (void)hipStreamSynchronize(sl->stream);
-        for (hipGraphExec_t g : sl->graph)
-            if (g)
-                (void)hipGraphExecDestroy(g);
-        if (sl->done)
-            (void)hipEventDestroy(sl->done);
+        for (StepBuf &b : sl->step) {
+            for (StepGraph &g : b.graph)
+                step_graph_destroy(g);
+            if (b.done)
+                (void)hipEventDestroy(b.done);
+            (void)hipFree(b.queue);
+        }
         if (sl->stream)
             (void)hipStreamDestroy(sl->stream);
-        (void)hipHostFree(sl->desc_host);
-        (void)hipFree(sl->desc_dev);
-        (void)hipFree(sl->queue);
         delete sl;
     }
     for (auto &kv : S->live) {
@@ -537,6 +570,7 @@ void dmap_server_destroy(dmap_server *S)
             (void)hipEventSynchronize(e);
             (void)hipEventDestroy(e);
         }
+    (void)hipFree(S->ray_table);
     (void)hipFree(S->arena_dev);
     delete S;
 }

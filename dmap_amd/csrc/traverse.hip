@@ -27,6 +27,7 @@
 #include <hip/hip_runtime.h>
 
 #include <atomic>
+#include <cstring>
 
 #include "kernels.h"
 #include "layout.h"
@@ -1270,9 +1271,11 @@ constexpr int MULTI_REFILL = 48;
 // lane at 3 and runs 1.9x faster at 2.
 template <class R, bool CN> constexpr int trace_waves() { return (sizeof(R) == sizeof(double) || !CN) ? 2 : 3; }
 
+// The walk of k_trace (single-object launches, sd NULL) and of
+// k_trace_multi (a serving step, sd its descriptor in the kernel's
+// arguments).
 template <class R, bool STATS, bool CN, bool TWO, bool MULTI>
-__global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(trace_waves<R, CN>(), 8)))
-k_trace(TraceParams p)
+__device__ __forceinline__ void trace_body(const TraceParams &p, const StepDesc *sd)
 {
 
     // The leaf search stacks: STACK entries per thread, stride blockDim.x
@@ -1285,12 +1288,16 @@ k_trace(TraceParams p)
     // per-triangle bookkeeping.
     const bool lds_bvh = !MULTI && p.obj.bvh_leaves <= LDS_BVH_LEAVES;
     unsigned char *bvh_lds = (unsigned char *)(s_code + STACK * blockDim.x);
-    // Multi-request launches: the step's request starts, staged in LDS
-    // (after the stacks) for the refill's binary search.
-    int64_t *s_seg_start = (int64_t *)bvh_lds;
+    // Multi-request launches: the step's segments, copied from the kernel
+    // arguments into LDS (after the stacks) for the refill's binary search
+    // and the lanes' request records.
+    Segment *s_seg = (Segment *)bvh_lds;
     if (MULTI) {
-        for (int q = threadIdx.x; q < p.step->n_seg; q += blockDim.x)
-            s_seg_start[q] = p.step->seg[q].start;
+        const uint32_t *src = (const uint32_t *)sd->seg;
+        uint32_t *dst = (uint32_t *)s_seg;
+        const int words = sd->n_seg * (int)(sizeof(Segment) / 4);
+        for (int q = threadIdx.x; q < words; q += blockDim.x)
+            dst[q] = src[q];
         __syncthreads();
     }
     if (lds_bvh) {
@@ -1316,10 +1323,10 @@ k_trace(TraceParams p)
     // Multi-request launches: the lane's request and its object.
     const Segment *SG = nullptr;
     const ObjectDev *O = nullptr;
-    const int64_t n_total = MULTI ? p.step->total : p.n_rays;
+    const int64_t n_total = MULTI ? sd->total : p.n_rays;
 #define DMAP_OBJ (MULTI ? *O : p.obj)
-#define DMAP_RAYS (MULTI ? SG->rays : p.rays)
-#define DMAP_HITS (MULTI ? SG->hits : p.hits)
+#define DMAP_RAYS (MULTI ? segment_rays(*SG) : p.rays)
+#define DMAP_HITS (MULTI ? segment_hits(*SG) : p.hits)
     R t_min = 0, t_best = 0;
     // The ray as given (fp32, so exact in R and in double).
     R o_f[3] = {0, 0, 0}, d_f[3] = {0, 0, 1}, inv_f[3] = {0, 0, 0};
@@ -1398,15 +1405,15 @@ k_trace(TraceParams p)
                         have = true;
                         if (MULTI) {
                             // The request holding this slot: a binary search of the starts.
-                            int lo = 0, hi = p.step->n_seg - 1;
+                            int lo = 0, hi = sd->n_seg - 1;
                             while (lo < hi) {
                                 int mid = (lo + hi + 1) >> 1;
-                                if (s_seg_start[mid] <= slot)
+                                if (s_seg[mid].start <= slot)
                                     lo = mid;
                                 else
                                     hi = mid - 1;
                             }
-                            SG = &p.step->seg[lo];
+                            SG = &s_seg[lo];
                             O = SG->obj;
                             rid = slot - SG->start + SG->first;
                         } else {
@@ -1605,7 +1612,27 @@ k_trace(TraceParams p)
             }
         }
     }
+}
 
+#undef DMAP_OBJ
+#undef DMAP_RAYS
+#undef DMAP_HITS
+
+// Closest hit / any hit of one object: persistent waves over the launch's
+// ray queue.
+template <class R, bool STATS, bool CN, bool TWO>
+__global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(trace_waves<R, CN>(), 8)))
+k_trace(TraceParams p)
+{
+    trace_body<R, STATS, CN, TWO, false>(p, nullptr);
+}
+
+// One serving step: the requests of `sd`, each against its own object.
+template <bool CN>
+__global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(trace_waves<float, CN>(), 8)))
+k_trace_multi(TraceParams p, StepDesc sd)
+{
+    trace_body<float, false, CN, false, true>(p, &sd);
 }
 
 // The reported hit, in double: the root polished (Newton leaf) or the face
@@ -1724,44 +1751,46 @@ __device__ __forceinline__ void finalize_one(const TraceParams &p, const Pyramid
                  H.texel_j[rid]);
 }
 
-template <bool MULTI> __global__ void __launch_bounds__(256) k_finalize(TraceParams p)
+// The hits of a closest-hit launch from the raw records its trace left.
+__global__ void __launch_bounds__(256) k_finalize(TraceParams p)
 {
     int64_t rid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const Segment *SG = nullptr;
-    if constexpr (MULTI) {
-        // Grid-stride over the step's requests (the grid is fixed for graph
-        // replay); the request starts are staged in LDS for the search.
-        const StepDesc &S = *p.step;
-        __shared__ int64_t s_start[MAX_SEGMENTS];
-        for (int q = threadIdx.x; q < S.n_seg; q += blockDim.x)
-            s_start[q] = S.seg[q].start;
-        __syncthreads();
-        for (int64_t slot = rid; slot < S.total; slot += (int64_t)gridDim.x * blockDim.x) {
-            int lo = 0, hi = S.n_seg - 1;
-            while (lo < hi) {
-                int mid = (lo + hi + 1) >> 1;
-                if (s_start[mid] <= slot)
-                    lo = mid;
-                else
-                    hi = mid - 1;
-            }
-            // Most waves lie within one request: read its record through a
-            // wave-uniform index (scalar loads) instead of lane by lane.
-            const int lo0 = __builtin_amdgcn_readfirstlane(lo);
-            if (__all(lo == lo0)) {
-                const Segment &U = S.seg[lo0];
-                const ObjectDev *uo = U.obj;
-                finalize_one(p, uo->pyr, uo->tris, U.rays, U.hits, slot - U.start + U.first);
-            } else {
-                SG = &S.seg[lo];
-                finalize_one(p, SG->obj->pyr, SG->obj->tris, SG->rays, SG->hits, slot - SG->start + SG->first);
-            }
-        }
-        return;
-    }
     if (rid >= p.n_rays)
         return;
     finalize_one(p, p.obj.pyr, p.obj.tris, p.rays, p.hits, rid);
+}
+
+// The same for a serving step: grid-stride over the step's slots (the grid
+// is fixed for graph replay), the segments staged in LDS for the search. It
+// runs while other slots' persistent traces hold most of the CUs, so its
+// waves are kept small (one-wave workgroups, at most 168 VGPRs for 3 waves
+// per SIMD): 256-thread workgroups at 182 VGPRs took twice as long there.
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 8)))
+k_finalize_multi(TraceParams p, StepDesc sd)
+{
+    __shared__ Segment s_seg[MAX_SEGMENTS];
+    {
+        const uint32_t *src = (const uint32_t *)sd.seg;
+        uint32_t *dst = (uint32_t *)s_seg;
+        const int words = sd.n_seg * (int)(sizeof(Segment) / 4);
+        for (int q = threadIdx.x; q < words; q += blockDim.x)
+            dst[q] = src[q];
+    }
+    __syncthreads();
+    for (int64_t slot = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; slot < sd.total;
+         slot += (int64_t)gridDim.x * blockDim.x) {
+        int lo = 0, hi = sd.n_seg - 1;
+        while (lo < hi) {
+            int mid = (lo + hi + 1) >> 1;
+            if (s_seg[mid].start <= slot)
+                lo = mid;
+            else
+                hi = mid - 1;
+        }
+        const Segment &G = s_seg[lo];
+        const ObjectDev *obj = G.obj;
+        finalize_one(p, obj->pyr, obj->tris, segment_rays(G), segment_hits(G), slot - G.start + G.first);
+    }
 }
 
 
@@ -2201,7 +2230,7 @@ template <class R, bool STATS, bool CN, bool TWO> static int resident_groups(siz
     static std::atomic<int> cached[2];
     int n = cached[with_bvh ? 1 : 0].load(std::memory_order_relaxed);
     if (n <= 0) {
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_trace<R, STATS, CN, TWO, false>, BLOCK, lds) !=
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_trace<R, STATS, CN, TWO>, BLOCK, lds) !=
                 hipSuccess ||
             n <= 0)
             n = TRACE_GROUPS_PER_CU;
@@ -2220,7 +2249,7 @@ static void launch_trace_k(const TraceParams &p, int64_t want, hipStream_t strea
     const size_t lds = trace_lds_bytes<R>(BLOCK) + bvh_lds;
     const int64_t cap = (int64_t)cu_count() * resident_groups<R, STATS, CN, TWO>(lds, bvh_lds > 0);
     const unsigned blocks = (unsigned)(want < cap ? want : cap);
-    hipLaunchKernelGGL((k_trace<R, STATS, CN, TWO, false>), dim3(blocks), dim3(BLOCK), lds, stream, p);
+    hipLaunchKernelGGL((k_trace<R, STATS, CN, TWO>), dim3(blocks), dim3(BLOCK), lds, stream, p);
 }
 
 template <class R, bool STATS, bool TWO>
@@ -2282,25 +2311,94 @@ hipError_t launch_trace(const TraceParams &p, int precision, hipStream_t stream,
     if (e != hipSuccess || p.any_hit)
         return e;
     unsigned fblocks = (unsigned)((p.n_rays + 255) / 256);
-    hipLaunchKernelGGL(k_finalize<false>, dim3(fblocks), dim3(256), 0, stream, p);
+    hipLaunchKernelGGL(k_finalize, dim3(fblocks), dim3(256), 0, stream, p);
     return hipGetLastError();
 }
 
-hipError_t launch_trace_multi(const TraceParams &p, bool const_normal, hipStream_t stream)
+// The step graph's launch shapes: as many trace waves as the persistent
+// single-object launch, in one-wave workgroups; a fixed finalize grid that
+// strides over the step.
+static unsigned multi_trace_blocks() { return (unsigned)cu_count() * 4 * (BLOCK / MULTI_BLOCK); }
+static size_t multi_trace_lds() { return trace_lds_bytes<float>(MULTI_BLOCK) + sizeof(Segment) * MAX_SEGMENTS; }
+constexpr unsigned MULTI_FINALIZE_BLOCKS = 4096;
+
+static void step_node_params(const StepGraph &g, bool trace, void **args, hipKernelNodeParams &kp)
 {
-    // As many waves as the persistent single-object launch.
-    unsigned blocks = (unsigned)cu_count() * 4 * (BLOCK / MULTI_BLOCK);
-    const size_t lds = trace_lds_bytes<float>(MULTI_BLOCK) + sizeof(int64_t) * MAX_SEGMENTS;
-    hipLaunchKernelGGL(k_queue_reset, dim3(1), dim3(1), 0, stream, p.queue);
-    // A serving step holds objects of one kind only (constant normals or
-    // interpolated ones), so it runs the same specialisation a single-object
-    // launch of those objects would.
-    if (const_normal)
-        hipLaunchKernelGGL((k_trace<float, false, true, false, true>), dim3(blocks), dim3(MULTI_BLOCK), lds, stream, p);
-    else
-        hipLaunchKernelGGL((k_trace<float, false, false, false, true>), dim3(blocks), dim3(MULTI_BLOCK), lds, stream, p);
-    hipLaunchKernelGGL(k_finalize<true>, dim3(blocks), dim3(MULTI_BLOCK), 0, stream, p);
-    return hipGetLastError();
+    std::memset(&kp, 0, sizeof kp);
+    if (trace) {
+        // A step holds objects of one kind only (constant normals or
+        // interpolated ones), so it runs the specialisation a single-object
+        // launch of those objects would.
+        kp.func = g.const_normal ? (void *)k_trace_multi<true> : (void *)k_trace_multi<false>;
+        kp.gridDim = dim3(multi_trace_blocks());
+        kp.blockDim = dim3(MULTI_BLOCK);
+        kp.sharedMemBytes = (unsigned)multi_trace_lds();
+    } else {
+        kp.func = (void *)k_finalize_multi;
+        kp.gridDim = dim3(MULTI_FINALIZE_BLOCKS);
+        kp.blockDim = dim3(MULTI_BLOCK);
+    }
+    kp.kernelParams = args;
+}
+
+hipError_t step_graph_create(unsigned *queue, bool const_normal, StepGraph &g)
+{
+    g = StepGraph{};
+    g.const_normal = const_normal;
+    hipError_t e = hipGraphCreate(&g.graph, 0);
+    if (e != hipSuccess)
+        return e;
+    // The queue reset takes the queue's address; the trace and finalize
+    // nodes get each step's parameters at launch (an empty step here).
+    hipGraphNode_t reset = nullptr;
+    hipKernelNodeParams kp;
+    std::memset(&kp, 0, sizeof kp);
+    void *reset_args[1] = {&queue};
+    kp.func = (void *)k_queue_reset;
+    kp.gridDim = dim3(1);
+    kp.blockDim = dim3(1);
+    kp.kernelParams = reset_args;
+    TraceParams p;
+    std::memset(&p, 0, sizeof p);
+    p.queue = queue;
+    StepDesc desc;
+    std::memset(&desc, 0, sizeof desc);
+    void *args[2] = {&p, &desc};
+    hipKernelNodeParams tp, fp;
+    step_node_params(g, true, args, tp);
+    step_node_params(g, false, args, fp);
+    if ((e = hipGraphAddKernelNode(&reset, g.graph, nullptr, 0, &kp)) != hipSuccess ||
+        (e = hipGraphAddKernelNode(&g.trace, g.graph, &reset, 1, &tp)) != hipSuccess ||
+        (e = hipGraphAddKernelNode(&g.finalize, g.graph, &g.trace, 1, &fp)) != hipSuccess ||
+        (e = hipGraphInstantiate(&g.exec, g.graph, nullptr, nullptr, 0)) != hipSuccess) {
+        step_graph_destroy(g);
+        return e;
+    }
+    return hipSuccess;
+}
+
+hipError_t step_graph_launch(StepGraph &g, const TraceParams &p, const StepDesc &desc, hipStream_t stream)
+{
+    TraceParams pp = p;
+    StepDesc dd = desc;
+    void *args[2] = {&pp, &dd};
+    hipKernelNodeParams tp, fp;
+    step_node_params(g, true, args, tp);
+    step_node_params(g, false, args, fp);
+    hipError_t e;
+    if ((e = hipGraphExecKernelNodeSetParams(g.exec, g.trace, &tp)) != hipSuccess ||
+        (e = hipGraphExecKernelNodeSetParams(g.exec, g.finalize, &fp)) != hipSuccess)
+        return e;
+    return hipGraphLaunch(g.exec, stream);
+}
+
+void step_graph_destroy(StepGraph &g)
+{
+    if (g.exec)
+        (void)hipGraphExecDestroy(g.exec);
+    if (g.graph)
+        (void)hipGraphDestroy(g.graph);
+    g = StepGraph{};
 }
 
 } // namespace dmap_amd

@@ -208,3 +208,37 @@ def test_pool_not_a_multiple_of_256(small_scenes):
         pass
     assert _same(_host(srv2.hits(0, n)), _host(scn.obj.intersect(rays)))
     srv2.release(0)
+
+
+def test_scattered_ray_planes_and_the_segment_cap(small_scenes):
+    """Requests whose eight ray arrays are separate allocations (not one
+    stride apart) go through the server's device record of their planes;
+    and 120 tiny requests fill steps up to their 48-request cap. Every
+    request's hits equal the single-request call's."""
+    import ctypes
+
+    import dmap_amd
+    import torch
+    scn = small_scenes[1]
+    srv = dmap_amd.Server(slots=3, step_rays=1 << 20, hit_pool_bytes=64 << 20)
+    n = 3000
+    rays = scn.primary[:, 4096:4096 + n].contiguous()
+    planes = [rays[q].clone() for q in range(8)]  # eight separate buffers
+    for q in range(8):
+        planes[q] = torch.cat([torch.zeros(q * 17 + 1, device="cuda"), planes[q]])[q * 17 + 1:]
+    r = dmap_amd._Rays(*[planes[q].data_ptr() for q in range(8)])
+    dmap_amd._check(dmap_amd._lib.dmap_server_submit(srv._h, scn.obj._h, ctypes.byref(r), n, ctypes.c_uint64(0)),
+                    "dmap_server_submit")
+    tiny = [scn.bounce[:, 100 * k:100 * k + 100].contiguous() for k in range(120)]
+    for k, t in enumerate(tiny):
+        srv.submit(scn.obj, t, 1 + k)
+    got = {}
+    while len(got) < 121:
+        for tag, _ in srv.poll():
+            got[tag] = _host(srv.hits(tag, n if tag == 0 else 100))
+            srv.release(tag)
+    torch.cuda.synchronize()
+    assert _same(got[0], _host(scn.obj.intersect(rays)))
+    for k, t in enumerate(tiny):
+        assert _same(got[1 + k], _host(scn.obj.intersect(t))), k
+    assert sum(int(np.isfinite(got[1 + k]["t"]).sum()) for k in range(120)) > 0
