@@ -30,6 +30,7 @@ DMAP_OK = 0
 BOUND = {"box": 0, "slab": 1, "both": 2}
 LEAF = {"newton": 0, "two_triangle": 1}
 PRECISION = {"fp32": 0, "fp64": 1}
+WALK_START = {"shell": 0, "roots": 1}
 BUILD = {"fold": 0, "direct": 1}
 CHANNELS = ("h0", "gu", "gv", "r", "ru", "rv", "h_min", "h_max")
 STAT_NAMES = ("triangle_tests", "node_tests", "leaf_tests", "sub_squares", "newton_runs", "newton_iters",
@@ -53,7 +54,8 @@ class DmapError(RuntimeError):
 
 class _Options(ctypes.Structure):
     _fields_ = [("bound", ctypes.c_int), ("slab_max_level", ctypes.c_int), ("leaf", ctypes.c_int),
-                ("precision", ctypes.c_int), ("reorder", ctypes.c_int), ("polish", ctypes.c_int)]
+                ("precision", ctypes.c_int), ("reorder", ctypes.c_int), ("polish", ctypes.c_int),
+                ("walk_start", ctypes.c_int)]
 
 
 _FP = ctypes.c_void_p
@@ -155,10 +157,17 @@ class TraversalOptions:
     precision: str = "fp32"
     reorder: bool = True
     polish: bool = True  # False: report the search's own root, without the double polish
+    # "shell": a constant-normal triangle's walk starts at the finest block of
+    # cells holding the ray's clip to its height shell (the same hits, fewer
+    # node tests); "roots": at the triangle's roots, as the reference's walk
+    # does, for work counters comparable with the reference's.
+    walk_start: str = "shell"
 
     def _c(self):
+        if self.walk_start not in WALK_START:
+            raise DmapError(f"walk_start {self.walk_start!r}: 'shell' or 'roots'")
         return _Options(BOUND[self.bound], int(self.slab_max_level), LEAF[self.leaf], PRECISION[self.precision],
-                        1 if self.reorder else 0, 1 if self.polish else 0)
+                        1 if self.reorder else 0, 1 if self.polish else 0, WALK_START[self.walk_start])
 
 
 class Context:

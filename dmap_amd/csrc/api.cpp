@@ -747,6 +747,8 @@ static int object_bounds(dmap_object *o, const std::vector<HostTri> &tris)
         if (!(lo <= hi))
             return fail(DMAP_ERR_INVALID, "object bounds: triangle %d's footprint holds no cell", k);
         triangle_box(tris[k], lo, hi, &o->boxes_host[6 * k]);
+        o->host[k].h_lo = lo;
+        o->host[k].h_hi = hi;
         for (int q = 0; q < 3; ++q) {
             o->host[k].aabb_lo[q] = o->boxes_host[6 * k + q];
             o->host[k].aabb_hi[q] = o->boxes_host[6 * k + 3 + q];
@@ -904,12 +906,13 @@ void dmap_object_destroy(dmap_object *obj)
 
 static int check_options(const dmap_options *opt, dmap_options &o)
 {
-    o = dmap_options{2, 1, 0, 0, 1, 1};
+    o = dmap_options{2, 1, 0, 0, 1, 1, 0};
     if (opt)
         o = *opt;
-    if (o.bound < 0 || o.bound > 2 || o.leaf < 0 || o.leaf > 1 || o.precision < 0 || o.precision > 1)
-        return fail(DMAP_ERR_INVALID, "bad traversal options (bound %d, leaf %d, precision %d)", o.bound, o.leaf,
-                    o.precision);
+    if (o.bound < 0 || o.bound > 2 || o.leaf < 0 || o.leaf > 1 || o.precision < 0 || o.precision > 1 ||
+        o.walk_start < 0 || o.walk_start > 1)
+        return fail(DMAP_ERR_INVALID, "bad traversal options (bound %d, leaf %d, precision %d, walk_start %d)",
+                    o.bound, o.leaf, o.precision, o.walk_start);
     return DMAP_OK;
 }
 
@@ -950,6 +953,7 @@ static int trace(const dmap_object *obj, const dmap_rays *rays, int64_t n, const
     p.slab_max_level = o.slab_max_level;
     p.leaf = o.leaf;
     p.polish = o.polish ? 1 : 0;
+    p.walk_start = o.walk_start;
     p.any_hit = occluded ? 1 : 0;
     p.n_rays = n;
     p.rays = RaysDev{rays->ox, rays->oy, rays->oz, rays->dx, rays->dy, rays->dz, rays->tmin, rays->tmax};

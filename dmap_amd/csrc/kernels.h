@@ -130,6 +130,7 @@ struct TraceParams
     int slab_max_level;
     int leaf; // 0 newton, 1 two-triangle
     int polish; // 1: the reported Newton root is polished in double (two steps)
+    int walk_start; // 0: constant-normal walks start at the shell's block (start_block); 1: at the roots
     int any_hit;
     int64_t n_rays;
     const uint32_t *perm; // lane order of the rays (NULL: as given)

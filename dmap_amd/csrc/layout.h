@@ -97,7 +97,11 @@ struct TriRecord
     double residual_tolerance; // 1e-10 x mean edge (displaced_intersector.cpp:150)
     float aabb_lo[3], aabb_hi[3];
     float mck[3]; // |mc| + mk per axis, rounded up
-    float pad_[1];
+    // The height range over the footprint (triangle_height_range,
+    // node_bounds.cpp:238-249), outward-rounded: every leaf texel the walk
+    // can reach has its heights inside it.
+    float h_lo, h_hi;
+    float pad_[3];
 };
 
 } // namespace dmap_amd

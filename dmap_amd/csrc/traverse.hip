@@ -1235,6 +1235,99 @@ __device__ uint32_t expand(const TraceParams &p, const PyramidView &P, const Tri
     return mask;
 }
 
+// Relative room of start_block's bounds, in units of the ray's magnitudes.
+constexpr double START_ROOM = 1e-5;
+
+// The finest 2 x 2 block of cells below a constant-normal triangle's root
+// block that holds every leaf where the walk can accept a hit of the ray
+// (tangent-space wk) within [t_min, t_max]. The surface over the footprint
+// lies in the shell {(u, v, 0) + h m}: (u, v) in the domain, h in the
+// footprint's height range [h_lo, h_hi], m the normal's tangent form
+// (centre mc, remainder mk per axis). A hit's z is h m_z, which clips the
+// ray to [t0, t1], and its (u, v) is the ray's (x, y) there less
+// h (m_x, m_y). A leaf accepts a surface point within its residual
+// tolerance of the ray (KNEWTON ulps of the residual's magnitude), so every
+// bound is widened by E, START_ROOM times the ray's magnitudes: above that
+// tolerance and the rounding of the few operations here (each within an
+// ulp of the same magnitudes) by two orders or more.
+// Returns 0 when the ray cannot meet the shell, 1 with the block set (its
+// lowest corner (bi, bj) at `level`, `valid` its cells that meet the
+// bounds, bit io + 2 jo), 2 when no block below the root block holds the
+// bounds (or they are unbounded): the walk starts at the roots.
+// The walk from the block meets every leaf where the walk from the roots
+// accepts a hit: a cell outside the bounds holds none, and a cell test only
+// prunes.
+template <class R>
+__device__ inline int start_block(const TriRecord &T, int n_leaf, const Walk<R> &wk, R t_min, R t_max, int &level,
+                                  int &bi, int &bj, uint32_t &valid)
+{
+    // (nmax and nmin pass NaNs on: every value below is kept finite.)
+    if (!isfinite(wk.inv[2]))
+        return 2;
+    const R room = (R)START_ROOM;
+    const R hl = (R)T.h_lo, hh = (R)T.h_hi, ah = nmax(fabs(hl), fabs(hh));
+    R lo[3], hi[3];
+    for (int q = 0; q < 3; ++q) {
+        const R mc = sizeof(R) == 8 ? (R)T.mc_d[q] : (R)T.mc[q], mk = sizeof(R) == 8 ? (R)T.mk_d[q] : (R)T.mk[q];
+        const R a = hl * mc, b = hh * mc;
+        lo[q] = nmin(a, b) - ah * mk;
+        hi[q] = nmax(a, b) + ah * mk;
+    }
+    const R inv = wk.inv[2];
+    R ta = (lo[2] - wk.o[2]) * inv, tb = (hi[2] - wk.o[2]) * inv;
+    if (ta > tb) {
+        R x = ta;
+        ta = tb;
+        tb = x;
+    }
+    const R zmag = fabs(lo[2]) + fabs(hi[2]);
+    const R dt = room * (wk.omag + wk.dmag * nmax(fabs(ta), fabs(tb)) + zmag) * fabs(inv);
+    const R t0 = nmax(t_min, ta - dt), t1 = nmin(t_max, tb + dt);
+    if (!(t0 <= t1))
+        return 0;
+    if (!(isfinite(t0) && isfinite(t1)))
+        return 2;
+    const R E = room * (wk.omag + wk.dmag * nmax(fabs(t0), fabs(t1)) + zmag);
+    // The root block's cells that meet the domain's bounding box.
+    const R W = ldexp((R)1, T.root_level) / (R)n_leaf;
+    int ri0 = T.root_i[0], ri1 = T.root_i[0], rj0 = T.root_j[0], rj1 = T.root_j[0];
+    for (int r = 1; r < T.n_roots; ++r) {
+        ri0 = min(ri0, T.root_i[r]);
+        ri1 = max(ri1, T.root_i[r]);
+        rj0 = min(rj0, T.root_j[r]);
+        rj1 = max(rj1, T.root_j[r]);
+    }
+    const R xa = wk.o[0] + t0 * wk.d[0], xb = wk.o[0] + t1 * wk.d[0];
+    const R ya = wk.o[1] + t0 * wk.d[1], yb = wk.o[1] + t1 * wk.d[1];
+    const R blo_x = sizeof(R) == 8 ? (R)T.dlo_d[0] : (R)T.dlo[0], bhi_x = sizeof(R) == 8 ? (R)T.dhi_d[0] : (R)T.dhi[0];
+    const R blo_y = sizeof(R) == 8 ? (R)T.dlo_d[1] : (R)T.dlo[1], bhi_y = sizeof(R) == 8 ? (R)T.dhi_d[1] : (R)T.dhi[1];
+    const R u0 = nmax(nmax(nmin(xa, xb) - hi[0], blo_x) - E, (R)ri0 * W);
+    const R u1 = nmin(nmin(nmax(xa, xb) - lo[0], bhi_x) + E, (R)(ri1 + 1) * W);
+    const R v0 = nmax(nmax(nmin(ya, yb) - hi[1], blo_y) - E, (R)rj0 * W);
+    const R v1 = nmin(nmin(nmax(ya, yb) - lo[1], bhi_y) + E, (R)(rj1 + 1) * W);
+    if (!(u0 <= u1 && v0 <= v1))
+        return 0;
+    // An interval no longer than a cell crosses at most one cell boundary:
+    // the level whose cells are at least the longer side fits, and the one
+    // below it may.
+    const R span = nmax(u1 - u0, v1 - v0) * (R)n_leaf; // in leaf texels
+    int L = span < (R)1 ? 0 : max(0, ilogb(span));
+    for (; L < T.root_level; ++L) {
+        const R s = ldexp((R)n_leaf, -L); // cells per unit
+        const R i0 = floor(u0 * s), i1 = nmax(i0, ceil(u1 * s) - (R)1);
+        const R j0 = floor(v0 * s), j1 = nmax(j0, ceil(v1 * s) - (R)1);
+        if (i1 - i0 <= (R)1 && j1 - j0 <= (R)1) {
+            level = L;
+            bi = (int)i0;
+            bj = (int)j0;
+            const bool wi = i1 > i0, wj = j1 > j0;
+            valid = 1u | (wi ? 2u : 0u) | (wj ? 4u : 0u) | (wi && wj ? 8u : 0u);
+            return 1;
+        }
+    }
+    return 2;
+}
+
 enum
 {
     ST_NEED = 0,   // bookkeeping: find the next block to expand or leaf to test
@@ -1544,6 +1637,21 @@ __device__ __forceinline__ void trace_body(const TraceParams &p, const StepDesc 
             uint32_t rv = 0;
             for (int r = 0; r < T->n_roots; ++r)
                 rv |= 1u << ((T->root_i[r] - rb_i) + 2 * (T->root_j[r] - rb_j));
+            if (T->const_normal && p.walk_start == 0) {
+                // Under a constant normal the walk starts at the finest block
+                // that holds the ray's possible hits.
+                int sl = 0, sbi = 0, sbj = 0;
+                uint32_t sv = 0;
+                int how = start_block<R>(*T, DMAP_OBJ.pyr.n_leaf, wk, t_min, t_best, sl, sbi, sbj, sv);
+                if (how == 0)
+                    continue; // no hit on this triangle: on to the next
+                if (how == 1) {
+                    root_level = sl;
+                    rb_i = sbi;
+                    rb_j = sbj;
+                    rv = sv;
+                }
+            }
             ex_level = root_level;
             ex_i = rb_i;
             ex_j = rb_j;

@@ -68,6 +68,12 @@ typedef struct
     int polish;         /* 1 (default): the reported root is polished by two Newton steps in
                            double; 0 reports the search's own root (precision's arithmetic),
                            evaluated in double, to measure the fp32 solve's error */
+    int walk_start;     /* 0 (default): a constant-normal triangle's walk starts at the finest
+                           2 x 2 block of cells that holds the ray's clip to the triangle's
+                           height shell (the same hits, fewer node tests); 1: at the
+                           triangle's root cells, as the reference's walk does
+                           (DisplacedIntersector::intersect, displaced_intersector.cpp:
+                           175-237), for work counters comparable with the reference's */
 } dmap_options;
 
 /* Device pointers, n entries each. */

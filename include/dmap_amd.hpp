@@ -52,10 +52,12 @@ struct TraversalOptions
     bool fp64 = false;   // the same solve in double on the device
     bool reorder = true; // sort rays for wave coherence (results are per ray either way)
     bool polish = true;  // polish the reported root in double (false: the search's own root)
+    bool from_roots = false; // every walk starts at the triangle's roots, as the reference's does
 
     dmap_options c() const
     {
-        return dmap_options{(int)bound, slab_max_level, (int)leaf, fp64 ? 1 : 0, reorder ? 1 : 0, polish ? 1 : 0};
+        return dmap_options{(int)bound, slab_max_level, (int)leaf, fp64 ? 1 : 0, reorder ? 1 : 0, polish ? 1 : 0,
+                            from_roots ? 1 : 0};
     }
 };
 

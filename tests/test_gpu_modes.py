@@ -167,3 +167,65 @@ def test_unpolished_fp32_root_meets_the_bar(torch_gpu, kind):
     both = np.isfinite(raw["t"])
     assert (both == np.isfinite(pol["t"])).all()
     assert not np.array_equal(raw["t"][both], pol["t"][both])  # the polish does move the reported t
+
+
+def _tilted_flat_quad():
+    """rotated_chart_quad (constant normal, rotated chart) with its positions
+    and normals turned about a skew axis and shifted: a tangent frame that is
+    no axis permutation."""
+    tris = scenes.rotated_chart_quad()
+    a = np.array([1.0, 2.0, 0.5]) / np.linalg.norm([1.0, 2.0, 0.5])
+    K = np.array([[0, -a[2], a[1]], [a[2], 0, -a[0]], [-a[1], a[0], 0]])
+    R = np.eye(3) + np.sin(0.7) * K + (1 - np.cos(0.7)) * K @ K
+    for row in tris:
+        for v in range(3):
+            row[3 * v:3 * v + 3] = R @ row[3 * v:3 * v + 3] + np.array([0.3, -0.2, 0.5])
+            row[9 + 3 * v:12 + 3 * v] = R @ row[9 + 3 * v:12 + 3 * v]
+    return tris
+
+
+@pytest.mark.parametrize("case", ["c2", "c2_fp64", "tilted", "span", "offset_clamp", "box"])
+def test_walk_start_does_not_change_results(torch_gpu, case):
+    """A constant-normal triangle's walk from the finest block that holds
+    the ray's clip to its height shell (the default) finds the same hits,
+    bit for bit, as the walk from its roots (walk_start "roots", the
+    reference's walk), with fewer node tests: on the incoherent workload's
+    structure (fp32 and fp64), a tilted frame with a rotated chart, a chart
+    over 6 x 6 repeats of the tile from negative coordinates, negative
+    heights with clamping, and the box-only bound."""
+    import dmap_amd
+    opt = {}
+    if case in ("c2", "c2_fp64"):
+        wl = W.c2_incoherent(n=256, width=128, height=128, n_bounce=16384)
+        tex, strength, offset, repeat, tris, rays = wl.texels, 0.1, 0.0, True, wl.triangles, wl.rays
+        if case == "c2_fp64":
+            opt["precision"] = "fp64"
+    else:
+        tex = np.random.default_rng(21).random((64, 64)).astype(np.float32)
+        strength, offset, repeat = 0.06, 0.0, True
+        tris = {"tilted": _tilted_flat_quad, "span": lambda: scenes.span_quad(-2.5, 3.5)}.get(
+            case, lambda: W.quad_plane(1, 1.0))()
+        if case == "offset_clamp":
+            offset, repeat = -0.05, False
+        if case == "box":
+            opt["bound"] = "box"
+        rays = scenes.rays(40000, 23)
+        if case == "tilted":
+            # The scene's rays, carried into the tilted quad's frame.
+            rays = rays.copy()
+            a = np.array([1.0, 2.0, 0.5]) / np.linalg.norm([1.0, 2.0, 0.5])
+            K = np.array([[0, -a[2], a[1]], [a[2], 0, -a[0]], [-a[1], a[0], 0]])
+            R = np.eye(3) + np.sin(0.7) * K + (1 - np.cos(0.7)) * K @ K
+            rays[0:3] = (R @ rays[0:3].astype(np.float64) + np.array([[0.3], [-0.2], [0.5]])).astype(np.float32)
+            rays[3:6] = (R @ rays[3:6].astype(np.float64)).astype(np.float32)
+    obj = build(tex, strength, offset, repeat, tris)
+    shell = gpu_hits(obj, rays, options=dmap_amd.TraversalOptions(**opt), stats=True)
+    roots = gpu_hits(obj, rays, options=dmap_amd.TraversalOptions(walk_start="roots", **opt), stats=True)
+    hits = int(np.isfinite(shell["t"]).sum())
+    ns, nr = int(shell["stats"]["node_tests"].sum()), int(roots["stats"]["node_tests"].sum())
+    print(f"[walk start] {case}: {hits} hits of {rays.shape[1]}; node tests {ns} from the shell's block, "
+          f"{nr} from the roots", flush=True)
+    assert hits > rays.shape[1] // 10
+    for key in ("t", "prim", "st", "bary", "ng", "texel"):
+        assert np.array_equal(shell[key], roots[key]), key
+    assert ns < nr
