@@ -443,15 +443,16 @@ extern "C" {
 const char *dmap_last_error(void) { return g_error.c_str(); }
 const char *dmap_version(void) { return "dmap_amd 0.1 gfx950"; }
 
-// Scratch of dmap_intersect_closest_noalloc: the ray-queue head (256 B) and
-// the ray reorder buffers (sort keys, permutation, radix sort temporary
-// storage; about 24 B per ray) of one call of at most NOALLOC_MAX_RAYS rays.
+// Scratch of dmap_intersect_closest_noalloc: the ray-queue head (256 B), the
+// ray reorder buffers (sort keys, permutation, radix sort temporary storage;
+// about 24 B per ray) and the raw records (32 B per ray) of one call of at
+// most NOALLOC_MAX_RAYS rays.
 // Every device the library loads on holds one copy for the life of the
 // process. Host code must not use `scratch` itself as an address: its
 // device address comes from hipGetSymbolAddress, per device, in
 // dmap_ctx_create, and is kept in dmap_ctx::scratch.
 constexpr int64_t NOALLOC_MAX_RAYS = (int64_t)1 << 20;
-constexpr size_t NOALLOC_SCRATCH_BYTES = (size_t)32 << 20;
+constexpr size_t NOALLOC_SCRATCH_BYTES = (size_t)64 << 20;
 static __device__ __align__(256) unsigned char scratch[NOALLOC_SCRATCH_BYTES];
 
 int dmap_ctx_create(int device, dmap_ctx **out)
@@ -971,16 +972,21 @@ static int trace(const dmap_object *obj, const dmap_rays *rays, int64_t n, const
     // Per-call workspace, stream-ordered: the ray queue head, and the
     // reorder buffers when asked for.
     bool reorder = o.reorder && n >= 4096;
-    size_t rbytes = reorder ? reorder_workspace_bytes(n) : 0;
+    size_t rbytes = reorder ? (reorder_workspace_bytes(n) + 255) / 256 * 256 : 0;
+    // A closest-hit call's raw records, after the queue head and the reorder
+    // buffers.
+    size_t raw_bytes = occluded ? 0 : sizeof(RawHit) * (size_t)n;
     void *ws = fixed_ws;
     if (fixed_ws) {
-        if (256 + rbytes > fixed_bytes)
+        if (256 + rbytes + raw_bytes > fixed_bytes)
             return fail(DMAP_ERR_INVALID, "intersect: %lld rays need %zu B of scratch, more than its %zu B",
-                        (long long)n, 256 + rbytes, fixed_bytes);
+                        (long long)n, 256 + rbytes + raw_bytes, fixed_bytes);
     } else {
-        HIP_TRY(hipMallocAsync(&ws, 256 + rbytes, s));
+        HIP_TRY(hipMallocAsync(&ws, 256 + rbytes + raw_bytes, s));
     }
     p.queue = (unsigned *)ws;
+    if (raw_bytes)
+        p.raw = (RawHit *)((char *)ws + 256 + rbytes);
     if (reorder) {
         uint32_t *perm = nullptr;
         hipError_t e = reorder_rays(p.rays, n, obj->bound_lo, obj->bound_hi, (char *)ws + 256, rbytes, &perm, s);

@@ -122,6 +122,18 @@ struct StepDesc
     Segment seg[MAX_SEGMENTS];
 };
 
+// A closest-hit trace's result for one ray, one aligned 32-byte sector:
+// k_trace writes it whole at the ray's index (the lanes of a wave hold rays
+// far apart in index, so the six 4-byte fields stored into separate arrays
+// would each write a fraction of a sector) and k_finalize reads it back in
+// index order. a, b: the root in texel-local coordinates.
+struct alignas(32) RawHit
+{
+    float t;
+    int32_t prim, ti, tj;
+    float a, b, pad_[2];
+};
+
 struct TraceParams
 {
     ObjectDev obj;        // single-object launches
@@ -137,6 +149,7 @@ struct TraceParams
     unsigned *queue;      // the launch's ray queue head (reset by launch_trace)
     RaysDev rays;
     HitsDev hits;
+    RawHit *raw; // n_rays records for k_finalize (NULL: the raw record goes into the hit arrays)
     uint8_t *occluded;
     StatsDev stats;
 };

@@ -1449,8 +1449,14 @@ __device__ __forceinline__ void trace_body(const TraceParams &p, const StepDesc 
             // The result of the finished ray.
             if (p.any_hit) {
                 p.occluded[rid] = hit_tri >= 0 ? 1 : 0;
+            } else if (!MULTI && p.raw) {
+                // The raw record, one whole sector; k_finalize turns it into the hit.
+                float4 *dst = (float4 *)(p.raw + rid);
+                dst[0] = make_float4(hit_tri >= 0 ? (float)t_best : INFINITY, __int_as_float(hit_tri),
+                                     __int_as_float(hit_i), __int_as_float(hit_j));
+                dst[1] = make_float4((float)hit_a, (float)hit_b, 0.f, 0.f);
             } else {
-                // The raw record; k_finalize turns it into the hit.
+                // The raw record in the hit arrays; k_finalize turns it into the hit.
                 const HitsDev &H = DMAP_HITS;
                 H.t[rid] = hit_tri >= 0 ? (float)t_best : INFINITY;
                 H.prim[rid] = hit_tri;
@@ -1865,7 +1871,14 @@ __global__ void __launch_bounds__(256) k_finalize(TraceParams p)
     int64_t rid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (rid >= p.n_rays)
         return;
-    finalize_one(p, p.obj.pyr, p.obj.tris, p.rays, p.hits, rid);
+    if (p.raw) {
+        const float4 *src = (const float4 *)(p.raw + rid);
+        const float4 r0 = src[0], r1 = src[1];
+        finalize_hit(p, p.obj.pyr, p.obj.tris, p.rays, p.hits, rid, __float_as_int(r0.y), r0.x, r1.x, r1.y,
+                     __float_as_int(r0.z), __float_as_int(r0.w));
+    } else {
+        finalize_one(p, p.obj.pyr, p.obj.tris, p.rays, p.hits, rid);
+    }
 }
 
 // The same for a serving step: grid-stride over the step's slots (the grid

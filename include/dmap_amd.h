@@ -171,8 +171,8 @@ int dmap_intersect_closest(const dmap_object *obj, const dmap_rays *rays, int64_
  * reference, which allocates nothing per ray; this is for callers that
  * cannot allocate device memory: code under HIP graph capture, tight
  * serving loops, harnesses that forbid hipMalloc). Same arguments and
- * results; its workspace (ray-queue head and ray reorder buffers) is a
- * fixed 32 MiB device array per GPU, held from library load to process
+ * results; its workspace (ray-queue head, ray reorder buffers and raw
+ * records) is a fixed 64 MiB device array per GPU, held from library load to process
  * exit, so one call takes at most 1,048,576 rays (larger n returns
  * DMAP_ERR_INVALID). Calls that overlap on one GPU share that array and
  * race: serialize them (one stream, or events). Kernel timing
