@@ -75,7 +75,18 @@ class _UvMap(ctypes.Structure):
 
 
 class _Stats(ctypes.Structure):
-    _fields_ = [(n, _FP) for n in STAT_NAMES] + [("node_tests_per_level", _FP)]
+    _fields_ = [(n, _FP) for n in STAT_NAMES] + [("node_tests_per_level", _FP), ("records", ctypes.c_void_p),
+                                                 ("record_capacity", ctypes.c_int64),
+                                                 ("record_count", ctypes.c_void_p)]
+
+
+# dmap_node_record (include/dmap_amd.h): one node test of the walk,
+# NodeTestRecord (displaced_intersector.h:77-87) with its ray, triangle,
+# cell and place among the ray's node tests.
+NODE_RECORD = np.dtype([("ray", np.int32), ("prim", np.int32), ("level", np.int32), ("i", np.int32),
+                        ("j", np.int32), ("seq", np.int32), ("box_pass", np.int32), ("slab_pass", np.int32),
+                        ("box_length", np.float32), ("slab_length", np.float32), ("both_length", np.float32),
+                        ("pad_", np.float32)])
 
 
 class _PtScene(ctypes.Structure):
@@ -303,9 +314,13 @@ class ObjectIntersector:
         return {"f": torch.empty((8, n), dtype=torch.float32, device="cuda"),
                 "i": torch.empty((3, n), dtype=torch.int32, device="cuda")}
 
-    def intersect(self, rays, options=None, prim_mask=-1, stats=False, out=None, stream=None):
+    def intersect(self, rays, options=None, prim_mask=-1, stats=False, out=None, stream=None, records=0):
         """Closest hit of each ray; returns device tensors t, st, bary, ng,
-        prim, texel (and the per-ray work counters when stats is set)."""
+        prim, texel (and the per-ray work counters when stats is set). With
+        records = capacity > 0 (and stats), also every node test of the
+        walks (TraversalStats::records): res["records"], a host array of
+        NODE_RECORD (the first `capacity` appended; rays interleave, each
+        ray's in seq order), and res["record_count"], how many were made."""
         torch = _torch()
         r, n = self._rays(rays)
         hits = out or self.alloc_hits(n)
@@ -313,9 +328,15 @@ class ObjectIntersector:
         h = _Hits(*[F[q].data_ptr() for q in range(8)], *[I[q].data_ptr() for q in range(3)])
         st = None
         stats_t = None
+        if records and not stats:
+            raise DmapError("node-test records come with the work counters: set stats=True")
         if stats:
             stats_t = torch.zeros((len(STAT_NAMES) + STAT_LEVELS, n), dtype=torch.int32, device="cuda")
             st = _Stats(*[stats_t[q].data_ptr() for q in range(len(STAT_NAMES) + 1)])
+            if records:
+                rec_t = torch.zeros((int(records), NODE_RECORD.itemsize // 4), dtype=torch.int32, device="cuda")
+                count_t = torch.zeros(1, dtype=torch.int64, device="cuda")
+                st.records, st.record_capacity, st.record_count = rec_t.data_ptr(), int(records), count_t.data_ptr()
         opt = (options or TraversalOptions())._c()
         _check(_lib.dmap_intersect_closest(self._h, ctypes.byref(r), n, ctypes.byref(h), int(prim_mask),
                                            ctypes.byref(opt), ctypes.byref(st) if st is not None else None,
@@ -324,6 +345,10 @@ class ObjectIntersector:
         if stats:
             res["stats"] = {k: stats_t[q] for q, k in enumerate(STAT_NAMES)}
             res["stats"]["node_tests_per_level"] = stats_t[len(STAT_NAMES):]
+        if records:
+            count = int(count_t.item())
+            res["record_count"] = count
+            res["records"] = rec_t[:min(count, int(records))].cpu().numpy().view(NODE_RECORD).reshape(-1)
         return res
 
     def pathtrace(self, cam_o, cam_d, spp, depth, light, radiance, env, albedo, seed=0, strategy="mis",

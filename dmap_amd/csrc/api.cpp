@@ -438,6 +438,9 @@ int bvh_leaves_for(int n_tri)
 
 } // namespace
 
+static_assert(sizeof(dmap_node_record) == sizeof(NodeRecordDev) && sizeof(NodeRecordDev) == 48,
+              "dmap_node_record and NodeRecordDev share one layout");
+
 extern "C" {
 
 const char *dmap_last_error(void) { return g_error.c_str(); }
@@ -962,10 +965,16 @@ static int trace(const dmap_object *obj, const dmap_rays *rays, int64_t n, const
         p.hits = HitsDev{hits->t,  hits->s,  hits->tt,   hits->bu,      hits->bv,     hits->nx,
                          hits->ny, hits->nz, hits->prim, hits->texel_i, hits->texel_j};
     p.occluded = occluded;
-    if (stats)
+    if (stats) {
+        if ((stats->records == nullptr) != (stats->record_count == nullptr) || stats->record_capacity < 0)
+            return fail(DMAP_ERR_INVALID, "intersect: node-test records need records, record_count and a "
+                                          "capacity >= 0 together");
         p.stats = StatsDev{stats->triangle_tests,       stats->node_tests,          stats->leaf_tests,
                            stats->sub_squares,          stats->newton_runs,         stats->newton_iters,
-                           stats->node_tests_after_hit, stats->leaf_tests_after_hit, stats->node_tests_per_level};
+                           stats->node_tests_after_hit, stats->leaf_tests_after_hit, stats->node_tests_per_level,
+                           reinterpret_cast<NodeRecordDev *>(stats->records), stats->record_capacity,
+                           reinterpret_cast<unsigned long long *>(stats->record_count)};
+    }
     hipStream_t s = (hipStream_t)stream;
     if (n > 0x7fffffffLL)
         return fail(DMAP_ERR_INVALID, "intersect: at most 2^31 - 1 rays per call");

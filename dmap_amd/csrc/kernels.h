@@ -40,11 +40,21 @@ struct HitsDev
     float *t, *s, *tt, *bu, *bv, *nx, *ny, *nz;
     int32_t *prim, *texel_i, *texel_j;
 };
+// dmap_node_record's layout (include/dmap_amd.h), device side.
+struct NodeRecordDev
+{
+    int32_t ray, prim, level, i, j, seq;
+    int32_t box_pass, slab_pass;
+    float box_length, slab_length, both_length, pad_;
+};
 struct StatsDev
 {
     int32_t *triangle_tests, *node_tests, *leaf_tests, *sub_squares, *newton_runs, *newton_iters;
     int32_t *node_tests_after_hit, *leaf_tests_after_hit;
     int32_t *node_tests_per_level; // STAT_LEVELS planes of n_rays, level-major
+    NodeRecordDev *records;        // NULL: no node-test records
+    int64_t record_capacity;
+    unsigned long long *record_count;
 };
 constexpr int STAT_LEVELS = 16; // levels 0 .. MAX_ROOT_LEVEL
 

@@ -377,11 +377,14 @@ template <class R> __device__ inline void cell_span(int k, R w, R lo, R hi, R &c
 // node_bound + node_interval (node_bounds.cpp:83-109, 111-211) for cell
 // (level, ci, cj), clipped to the triangle's texture bounding box as centre
 // (cx, cy) and half-widths (hx, hy) (cell_span), with its min-max pair mm
-// and, when the slab is read, its Taylor quad tp. Returns whether the ray
-// can meet the surface over the cell within [t_min, t_max].
+// and, when the slab is read, its Taylor quad tp; bound 0 box, 1 slab,
+// 2 both, the slab read when read_slab (else the box alone). Returns
+// whether the ray can meet the surface over the cell within [t_min, t_max],
+// and then the interval in [r0, r1].
 template <class R, bool CN>
-__device__ bool cell_test(const TraceParams &p, const PyramidView &P, const TriRecord &T, const Walk<R> &wk, int level,
-                          int ci, int cj, R cx, R cy, R hx, R hy, float2 mm, float4 tp, R t_min, R t_max)
+__device__ bool cell_interval(int bound, bool read_slab, const PyramidView &P, const TriRecord &T, const Walk<R> &wk,
+                              int level, int ci, int cj, R cx, R cy, R hx, R hy, float2 mm, float4 tp, R t_min,
+                              R t_max, R &r0, R &r1)
 {
     R w = ldexp((R)1, level - (__ffs(P.n_leaf) - 1)); // 2^level / n_leaf, n_leaf a power of two
 
@@ -419,9 +422,8 @@ __device__ bool cell_test(const TraceParams &p, const PyramidView &P, const TriR
         be[q] = e;
     }
 
-    bool read_slab = level <= p.slab_max_level;
-    bool want_box = p.bound == 0 || p.bound == 2 || !read_slab;
-    bool want_slab = p.bound != 0 && read_slab;
+    bool want_box = bound == 0 || bound == 2 || !read_slab;
+    bool want_slab = bound != 0 && read_slab;
 
     // The box's x and y clips are the column's (slab_interval,
     // node_bounds.cpp:137-179): both start from [t_min, t_max], so the two
@@ -434,8 +436,11 @@ __device__ bool cell_test(const TraceParams &p, const PyramidView &P, const TriR
     if (want_box) {
         if (!clip_node_box<R>(wk.o[2], wk.d[2], wk.inv[2], bc[2], be[2], b0, b1) || !close_interval<R>(b0, b1))
             return false;
-        if (!want_slab)
+        if (!want_slab) {
+            r0 = b0;
+            r1 = b1;
             return true;
+        }
     }
 
     // slab_interval: the column.
@@ -504,11 +509,67 @@ __device__ bool cell_test(const TraceParams &p, const PyramidView &P, const TriR
         s1 = x;
     }
     R st1 = t0 + s1, st0 = t0 + s0;
-    if (!want_box)
+    if (!want_box) {
+        r0 = st0;
+        r1 = st1;
         return true;
+    }
     // both_interval (node_bounds.cpp:181-197).
     R u0 = nmax(b0, st0), u1 = nmin(b1, st1);
-    return close_interval<R>(u0, u1);
+    if (!close_interval<R>(u0, u1))
+        return false;
+    r0 = u0;
+    r1 = u1;
+    return true;
+}
+
+// node_interval with the options' bound mode: whether the ray can meet the
+// surface over the cell within [t_min, t_max].
+template <class R, bool CN>
+__device__ inline bool cell_test(const TraceParams &p, const PyramidView &P, const TriRecord &T, const Walk<R> &wk,
+                                 int level, int ci, int cj, R cx, R cy, R hx, R hy, float2 mm, float4 tp, R t_min,
+                                 R t_max)
+{
+    R r0, r1;
+    return cell_interval<R, CN>(p.bound, level <= p.slab_max_level, P, T, wk, level, ci, cj, cx, cy, hx, hy, mm, tp,
+                                t_min, t_max, r0, r1);
+}
+
+// record_node_test (displaced_intersector.cpp:159-172) for one node test of
+// the walk: the box's and the slab's decisions with the walk's t_max, and
+// the three interval lengths with t_max = inf, the slab read at any level.
+// Appended at the next slot of p.stats.records (counted past its capacity,
+// not written).
+template <class R, bool CN>
+__device__ void record_node_test(const TraceParams &p, const PyramidView &P, const TriRecord &T, const Walk<R> &wk,
+                                 int level, int ci, int cj, R cx, R cy, R hx, R hy, float2 mm, R t_min, R t_max,
+                                 int64_t rid, int prim, int seq)
+{
+    const float4 tp = fetch_tp(P, level, ci, cj);
+    NodeRecordDev rec;
+    rec.ray = (int32_t)rid;
+    rec.prim = prim;
+    rec.level = level;
+    rec.i = ci;
+    rec.j = cj;
+    rec.seq = seq;
+    R r0, r1;
+    rec.box_pass = cell_interval<R, CN>(0, true, P, T, wk, level, ci, cj, cx, cy, hx, hy, mm, tp, t_min, t_max, r0, r1);
+    rec.slab_pass = cell_interval<R, CN>(1, true, P, T, wk, level, ci, cj, cx, cy, hx, hy, mm, tp, t_min, t_max, r0, r1);
+    const R inf = (R)INFINITY;
+    float len[3];
+    for (int mode = 0; mode < 3; ++mode)
+        len[mode] = cell_interval<R, CN>(mode, true, P, T, wk, level, ci, cj, cx, cy, hx, hy, mm, tp, t_min, inf, r0,
+                                         r1)
+                        ? (float)(r1 - r0)
+                        : -1.f;
+    rec.box_length = len[0];
+    rec.slab_length = len[1];
+    rec.both_length = len[2];
+    rec.pad_ = 0.f;
+    const unsigned long long slot = atomicAdd(p.stats.record_count, 1ull);
+    if (slot < (unsigned long long)p.stats.record_capacity)
+        p.stats.records[slot] = rec;
 }
 
 // ------------------------------------------------------------------ leaf
@@ -1184,9 +1245,13 @@ __device__ inline float4 group_tp(const Group &g, int q)
 // Returns the passing cells as a 4-bit mask in the order the ray meets
 // them: bit k is cell (bi + ((k >> 1) ^ si), bj + ((k & 1) ^ sj)), the
 // child order of cell_next (displaced_intersector.cpp:99-121).
-template <class R, bool CN>
+// REC: the launch counts work (STATS); then, when p.stats.records is set,
+// each node test is recorded for ray rid, triangle prim, its place in the
+// ray's node tests counted from seq0.
+template <class R, bool CN, bool REC>
 __device__ uint32_t expand(const TraceParams &p, const PyramidView &P, const TriRecord &T, const Walk<R> &wk, int level,
-                           int bi, int bj, bool aligned, uint32_t valid, bool &inside, R t_min, R t_max, int &node_tests)
+                           int bi, int bj, bool aligned, uint32_t valid, bool &inside, R t_min, R t_max, int &node_tests,
+                           int64_t rid, int prim, int seq0)
 {
     bool slab = p.bound != 0 && level <= p.slab_max_level;
     Group g;
@@ -1226,6 +1291,9 @@ __device__ uint32_t expand(const TraceParams &p, const PyramidView &P, const Tri
             continue;
         if (!inside && square_outside<R>(T, ((R)ci + (R)0.5) * w, ((R)cj + (R)0.5) * w, (R)0.5 * w))
             continue;
+        if (REC && p.stats.records)
+            record_node_test<R, CN>(p, P, T, wk, level, ci, cj, sx_c[io], sy_c[jo], sx_h[io], sy_h[jo],
+                                    group_mm(g, q), t_min, t_max, rid, prim, seq0 + node_tests);
         ++node_tests;
         float4 tp = slab ? g.tp[q] : make_float4(0.f, 0.f, 0.f, 0.f);
         if (cell_test<R, CN>(p, P, T, wk, level, ci, cj, sx_c[io], sy_c[jo], sx_h[io], sy_h[jo], group_mm(g, q), tp,
@@ -1669,8 +1737,8 @@ __device__ __forceinline__ void trace_body(const TraceParams &p, const StepDesc 
         if (state == ST_EXPAND) {
             int tests = 0;
             bool inside = ex_level < in_lvl;
-            uint32_t mask = expand<R, CN>(p, DMAP_OBJ.pyr, *T, wk, ex_level, ex_i, ex_j, ex_aligned, ex_valid, inside,
-                                          t_min, t_best, tests);
+            uint32_t mask = expand<R, CN, STATS>(p, DMAP_OBJ.pyr, *T, wk, ex_level, ex_i, ex_j, ex_aligned, ex_valid,
+                                                 inside, t_min, t_best, tests, rid, k, STATS ? st.node : 0);
             if (inside && in_lvl < 0)
                 in_lvl = ex_level;
             if (STATS) {
@@ -2412,7 +2480,7 @@ hipError_t launch_trace(const TraceParams &p, int precision, hipStream_t stream,
     hipLaunchKernelGGL(k_queue_reset, dim3(1), dim3(1), 0, stream, p.queue);
     bool stats = p.stats.node_tests || p.stats.leaf_tests || p.stats.sub_squares || p.stats.triangle_tests ||
                  p.stats.newton_runs || p.stats.newton_iters || p.stats.node_tests_after_hit ||
-                 p.stats.leaf_tests_after_hit || p.stats.node_tests_per_level;
+                 p.stats.leaf_tests_after_hit || p.stats.node_tests_per_level || p.stats.records;
     if (ev_start)
         (void)hipEventRecord(ev_start, stream);
     if (precision == 1) {

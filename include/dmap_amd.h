@@ -98,10 +98,34 @@ typedef struct
  * level-major ([level * n + ray]); a triangle's root level is at most 15,
  * so no node lies above the last plane. */
 #define DMAP_STAT_LEVELS 16
+
+/* One node test as the walk made it, NodeTestRecord (displaced_intersector.h:
+ * 77-87, record_node_test at displaced_intersector.cpp:159-172): the box's
+ * and the slab's decisions with the walk's current t_max, and the lengths
+ * of the box's, the slab's and both's intervals with t_max = inf (-1 when
+ * that test fails). The slab is read at every level here, whatever the
+ * options' slab_max_level, as the reference's record does. The cell
+ * (level, i, j), the ray's index in the call, the triangle and the test's
+ * place among the ray's node tests (seq, from 0) say which test it was;
+ * records of different rays interleave in the array. */
+typedef struct
+{
+    int32_t ray, prim, level, i, j, seq;
+    int32_t box_pass, slab_pass;
+    float box_length, slab_length, both_length, pad_;
+} dmap_node_record;
+
+/* records (device array of record_capacity entries) and record_count (a
+ * device uint64 the caller zeroes) may be NULL together; when set, every
+ * node test appends a record and adds one to *record_count, records past
+ * the capacity are counted and not written. */
 typedef struct
 {
     int32_t *triangle_tests, *node_tests, *leaf_tests, *sub_squares, *newton_runs, *newton_iters;
     int32_t *node_tests_after_hit, *leaf_tests_after_hit, *node_tests_per_level;
+    dmap_node_record *records;
+    int64_t record_capacity;
+    uint64_t *record_count;
 } dmap_ray_stats;
 
 const char *dmap_last_error(void);

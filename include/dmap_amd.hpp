@@ -15,6 +15,7 @@
 // arrays), not one ray, and run on the GPU.
 #pragma once
 
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -278,9 +279,15 @@ class ObjectIntersector
     }
 
     // Closest hits of host rays (ObjectIntersector::intersect over a batch;
-    // with stats, the per-ray work counters).
+    // with stats, the per-ray work counters). With records (and stats),
+    // every node test of the walks is appended to *records, as
+    // TraversalStats::records collects them (displaced_intersector.h:
+    // 101-102), up to record_capacity; rays interleave, each ray's records
+    // in seq order.
     std::vector<TraceResult> intersect(const std::vector<RayQuery> &rays, const TraversalOptions &options = {},
-                                       bool stats = false, int prim_mask = -1) const
+                                       bool stats = false, int prim_mask = -1,
+                                       std::vector<dmap_node_record> *records = nullptr,
+                                       int64_t record_capacity = 0) const
     {
         if (rays.empty())
             return {};
@@ -292,11 +299,20 @@ class ObjectIntersector
         const size_t n = rays.size();
         constexpr int planes = 6 + DMAP_STAT_LEVELS;
         DeviceArray<int32_t> counters(*ctx_, stats ? planes * n : 0);
+        const bool rec = stats && records && record_capacity > 0;
+        DeviceArray<dmap_node_record> rec_buf(*ctx_, rec ? (size_t)record_capacity : 0);
+        DeviceArray<uint64_t> rec_count(*ctx_, rec ? 1 : 0);
         dmap_ray_stats st{};
         if (stats) {
             int32_t *c = counters.data();
             st = dmap_ray_stats{c,       c + n,     c + 2 * n, c + 3 * n, nullptr,
                                 nullptr, c + 4 * n, c + 5 * n, c + 6 * n};
+        }
+        if (rec) {
+            rec_count.upload({0});
+            st.records = rec_buf.data();
+            st.record_capacity = record_capacity;
+            st.record_count = rec_count.data();
         }
         check(dmap_intersect_closest(h_, &rc, (int64_t)n, &hc, prim_mask, &o, stats ? &st : nullptr, nullptr),
               "dmap_intersect_closest");
@@ -314,6 +330,12 @@ class ObjectIntersector
                 for (int l = 0; l < DMAP_STAT_LEVELS; ++l)
                     out[q].stats.node_tests_per_level[l] = c[(6 + l) * n + q];
             }
+        }
+        if (rec) {
+            const uint64_t made = rec_count.download()[0];
+            std::vector<dmap_node_record> all = rec_buf.download();
+            all.resize((size_t)std::min<uint64_t>(made, (uint64_t)record_capacity));
+            records->insert(records->end(), all.begin(), all.end());
         }
         return out;
     }

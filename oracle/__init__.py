@@ -85,6 +85,8 @@ def _declare(L):
     L.orc_scene_intersect.argtypes = [_VP, c.c_int64, _D, _D, _D, _D, c.c_int, c.c_int, c.c_int, c.c_int,
                                       _D, _D, _D, _D, _I32, _I64, _I64]
     L.orc_scene_occluded.argtypes = [_VP, c.c_int64, _D, _D, _D, _D, c.c_int, c.c_int, c.c_int, c.c_int, _U8]
+    L.orc_scene_records.argtypes = [_VP, c.c_int64, _D, _D, _D, _D, c.c_int, c.c_int, c.c_int, c.c_int64, _VP,
+                                    _I64]
     L.orc_triangle_intersect.argtypes = [_VP, c.c_int, _D, _D, c.c_double, c.c_double, c.c_int, c.c_int,
                                          c.c_int, _D, _I64]
     L.orc_leaf_newton.argtypes = [_VP, c.c_int, _D, _D, c.c_double, c.c_double, c.c_int64, c.c_int64, _D, _I64]
@@ -108,6 +110,14 @@ def _declare(L):
 
 def _p(a, t=_D):
     return None if a is None else a.ctypes.data_as(t)
+
+
+# orc_node_record (dmap_oracle.h): NodeTestRecord (displaced_intersector.h:77-87)
+# with the ray, the triangle, the cell and the test's place among the ray's
+# node tests.
+NODE_RECORD = np.dtype([("ray", np.int32), ("prim", np.int32), ("level", np.int32), ("seq", np.int32),
+                        ("i", np.int64), ("j", np.int64), ("box_pass", np.int32), ("slab_pass", np.int32),
+                        ("box_length", np.float64), ("slab_length", np.float64), ("both_length", np.float64)])
 
 
 def _f64(a):
@@ -210,6 +220,21 @@ class Scene:
         if stats:
             out["stats"] = {k: st[:, q] for q, k in enumerate(STAT_NAMES)}
             out["stats"]["node_tests_per_level"] = st[:, len(STAT_NAMES):]
+        return out
+
+    def records(self, o, d, tmin=None, tmax=None, bound="both", slab_max_level=1, leaf="newton"):
+        """Every node test of the rays' closest-hit walks as NodeTestRecord
+        (record_node_test, displaced_intersector.cpp:159-172), ray by ray in
+        walk order: a structured array of NODE_RECORD."""
+        o, d = _f64(o).reshape(-1, 3), _f64(d).reshape(-1, 3)
+        n = o.shape[0]
+        tmin = np.zeros(n) if tmin is None else _f64(np.broadcast_to(tmin, (n,)))
+        tmax = np.full(n, np.inf) if tmax is None else _f64(np.broadcast_to(tmax, (n,)))
+        count = np.zeros(1, dtype=np.int64)
+        args = (self.handle, n, _p(o), _p(d), _p(tmin), _p(tmax), BOUND[bound], int(slab_max_level), LEAF[leaf])
+        lib().orc_scene_records(*args, 0, None, _p(count, _I64))
+        out = np.zeros(int(count[0]), dtype=NODE_RECORD)
+        lib().orc_scene_records(*args, len(out), out.ctypes.data_as(_VP), _p(count, _I64))
         return out
 
     def occluded(self, o, d, tmin=None, tmax=None, bound="both", slab_max_level=1, leaf="newton", threads=0):

@@ -920,7 +920,34 @@ struct Hit
 struct Stats
 {
     int64_t v[ORC_STATS] = {};
+    // When set, every node test of the walk is appended
+    // (TraversalStats::records, displaced_intersector.h:101-102), for ray
+    // `ray` on triangle `prim`.
+    std::vector<orc_node_record> *records = nullptr;
+    int32_t ray = 0, prim = 0;
 };
+
+// record_node_test (displaced_intersector.cpp:159-172): the box's and the
+// slab's decisions with the walk's current t_max, then the three interval
+// lengths with t_max = inf (-1 when that test fails).
+static void record_node_test(const NodeBound &nb, TRay ray, int level, int64_t i, int64_t j, Stats &st)
+{
+    orc_node_record rec{};
+    rec.ray = st.ray;
+    rec.prim = st.prim;
+    rec.level = level;
+    rec.seq = (int32_t)(st.v[ORC_ST_NODE] - 1);
+    rec.i = i;
+    rec.j = j;
+    double t0, t1;
+    rec.box_pass = box_interval(nb, ray, t0, t1);
+    rec.slab_pass = slab_interval(nb, ray, t0, t1);
+    ray.t_max = INFINITY;
+    rec.box_length = box_interval(nb, ray, t0, t1) ? t1 - t0 : -1.0;
+    rec.slab_length = slab_interval(nb, ray, t0, t1) ? t1 - t0 : -1.0;
+    rec.both_length = both_interval(nb, ray, t0, t1) ? t1 - t0 : -1.0;
+    st.records->push_back(rec);
+}
 
 // restricted (displaced_intersector.cpp:72-81).
 static AF restricted(const AF &f, double a, double b, double rho)
@@ -1179,6 +1206,8 @@ struct Intersector
                 }
                 NodeBound nb = node_bound(*tri, frame, *pyr, cur.level, cur.i, cur.j);
                 ray.t_max = t_max;
+                if (st && st->records)
+                    record_node_test(nb, ray, cur.level, cur.i, cur.j, *st);
                 double t0, t1;
                 if (!node_interval(nb, ray, opt, t0, t1)) {
                     cell_next(cur, si, sj, root_level);
@@ -1234,8 +1263,10 @@ struct Scene
             double t0 = t_min, t1 = t_max;
             if (!boxes[k].clip(o, d, t0, t1))
                 continue;
-            if (st)
+            if (st) {
                 ++st->v[ORC_ST_TRIANGLE];
+                st->prim = (int32_t)k;
+            }
             Hit h;
             if (its[k].intersect(o, d, t_min, t_max, opt, h, st)) {
                 hit = h;
@@ -1439,6 +1470,27 @@ void orc_scene_intersect(void *scene, int64_t n, const double *o, const double *
             for (int q = 0; q < ORC_STATS; ++q)
                 stats[ORC_STATS * r + q] = stt.v[q];
     });
+}
+
+void orc_scene_records(void *scene, int64_t n, const double *o, const double *d, const double *tmin,
+                       const double *tmax, int bound, int slab_max_level, int leaf, int64_t capacity,
+                       orc_node_record *out, int64_t *count)
+{
+    const Scene *s = (const Scene *)scene;
+    Options opt{bound, slab_max_level, leaf};
+    std::vector<orc_node_record> records;
+    for (int64_t r = 0; r < n; ++r) {
+        Hit h;
+        int tri = -1;
+        Stats stt;
+        stt.records = &records;
+        stt.ray = (int32_t)r;
+        s->intersect(v3(o[3 * r], o[3 * r + 1], o[3 * r + 2]), v3(d[3 * r], d[3 * r + 1], d[3 * r + 2]), tmin[r],
+                     tmax[r], opt, h, tri, &stt, false);
+    }
+    *count = (int64_t)records.size();
+    for (int64_t q = 0; q < std::min(capacity, *count); ++q)
+        out[q] = records[(size_t)q];
 }
 
 // ------------------------------------------------------------------ adjoint

@@ -70,6 +70,27 @@ void orc_scene_intersect(void *scene, int64_t n, const double *o, const double *
                          const double *tmax, int bound, int slab_max_level, int leaf, int nthreads, double *t,
                          double *st, double *bary, double *ng, int32_t *prim, int64_t *texel, int64_t *stats);
 
+/* One node test as the walk made it, NodeTestRecord (displaced_intersector.h:
+ * 77-87; record_node_test, displaced_intersector.cpp:159-172): the ray's
+ * index, the triangle, the cell, the test's place among the ray's node tests
+ * (seq, from 0), the box's and the slab's decisions with the walk's t_max,
+ * and the three interval lengths with t_max = inf (-1 when that test fails). */
+typedef struct
+{
+    int32_t ray, prim, level, seq;
+    int64_t i, j;
+    int32_t box_pass, slab_pass;
+    double box_length, slab_length, both_length;
+} orc_node_record;
+
+/* The records of every node test of the closest-hit walks of n rays, in
+ * order (ray by ray, one thread), as TraversalStats::records collects them
+ * (test_traversal_cost.cpp:377-386). Writes at most capacity records into
+ * out and the total into *count. */
+void orc_scene_records(void *scene, int64_t n, const double *o, const double *d, const double *tmin,
+                       const double *tmax, int bound, int slab_max_level, int leaf, int64_t capacity,
+                       orc_node_record *out, int64_t *count);
+
 /* Any-hit: the occluded callback's answer (displaced_geometry.cpp:79-106)
  * with no filter, over the flat loop with an early exit. */
 void orc_scene_occluded(void *scene, int64_t n, const double *o, const double *d, const double *tmin,

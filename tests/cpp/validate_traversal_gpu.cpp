@@ -164,6 +164,47 @@ int main()
             std::printf("(5) %s: node tests box %lld, both %lld\n", kind == 0 ? "flat" : "curved", (long long)nb,
                         (long long)nbo);
             pass = pass && nbo <= nb;
+            // (6) the node-test records of the box walk from the roots (TraversalStats::records): one per node
+            // test, each ray's numbered from 0, and every cell whose box test passed above the leaves has its
+            // children tested later in the same walk (in box mode the box's decision is the walk's).
+            {
+                std::vector<dmap_node_record> rec;
+                TraversalOptions roots = box;
+                roots.from_roots = true;
+                std::vector<TraceResult> sr = obj.intersect(rays, roots, true, -1, &rec, (int64_t)(4 * nb + 1024));
+                int64_t nr = 0;
+                for (const TraceResult &x : sr)
+                    nr += x.stats.node_tests;
+                std::vector<std::vector<dmap_node_record>> per(rays.size());
+                for (const dmap_node_record &x : rec)
+                    per[(size_t)x.ray].push_back(x);
+                int64_t bad = 0, descents = 0;
+                for (size_t r = 0; r < rays.size(); ++r) {
+                    std::vector<dmap_node_record> &v = per[r];
+                    std::sort(v.begin(), v.end(),
+                              [](const dmap_node_record &a, const dmap_node_record &b) { return a.seq < b.seq; });
+                    if ((int64_t)v.size() != sr[r].stats.node_tests)
+                        ++bad;
+                    for (size_t q = 0; q < v.size(); ++q) {
+                        if (v[q].seq != (int32_t)q)
+                            ++bad;
+                        if (!v[q].box_pass || v[q].level == 0)
+                            continue;
+                        ++descents;
+                        bool child = false;
+                        for (size_t u = q + 1; u < v.size() && !child; ++u)
+                            child = v[u].prim == v[q].prim && v[u].level == v[q].level - 1 &&
+                                    (v[u].i >> 1) == v[q].i && (v[u].j >> 1) == v[q].j;
+                        if (!child)
+                            ++bad;
+                    }
+                }
+                std::printf("(6) %s: %zu node-test records of %lld node tests, %lld passing cells above the "
+                            "leaves, %lld without their children or out of order\n",
+                            kind == 0 ? "flat" : "curved", rec.size(), (long long)nr, (long long)descents,
+                            (long long)bad);
+                pass = pass && (int64_t)rec.size() == nr && bad == 0 && descents > 0;
+            }
         }
         // (4) never-miss on the flat quad under a smooth map
         {

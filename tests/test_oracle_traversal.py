@@ -313,3 +313,80 @@ def test_work_counters_per_level_and_after_hit():
     assert (bs["leaf_tests_after_hit"][hit] <= bs["leaf_tests"][hit] - 1).all()
     # Some walks go on after their final hit (the child order is not front to back).
     assert bs["node_tests_after_hit"][hit].sum() > 0
+
+
+def _box_clip_length(lo, hi, o, d, t_min):
+    """The length of a ray's stretch inside the box [lo, hi] from t_min on
+    (-1 when it misses), by the slab method in numpy."""
+    t0, t1 = t_min, np.inf
+    for a in range(3):
+        if d[a] != 0.0:
+            ta, tb = sorted(((lo[a] - o[a]) / d[a], (hi[a] - o[a]) / d[a]))
+            t0, t1 = max(t0, ta), min(t1, tb)
+        elif not lo[a] <= o[a] <= hi[a]:
+            return -1.0
+    if t0 > t1 + 1e-9 * max(1.0, abs(t0), abs(t1)):
+        return -1.0
+    return abs(t1 - t0)
+
+
+@pytest.mark.parametrize("kind", ["flat", "curved"])
+def test_node_records_follow_the_walk(kind):
+    """NodeTestRecord (displaced_intersector.h:77-87, record_node_test at
+    displaced_intersector.cpp:159-172) along the box walk, the walk the
+    reference records (test_traversal_cost.cpp:377-386): one record per node
+    test, per ray and per level as TraversalStats counts them, each ray's in
+    seq order; the box's decision is the walk's (a passing cell with
+    children is followed by one of its children, a failing one by a cell
+    no finer); a test that passes with the walk's t_max passes with
+    t_max = inf; both's interval lies inside the box's and the slab's; and
+    the box length equals a numpy slab clip of the node's box with the ray
+    in the triangle's tangent frame."""
+    from tests import scenes
+    if kind == "flat":
+        sc, _, tris = small_scene(n=32, seed=3)
+    else:
+        tex = np.random.default_rng(4).random((32, 32)).astype(np.float32)
+        tris = scenes.curved_quad()
+        sc = oracle.Scene(W.close_tile(tex).astype(np.float64), 0.06, 0.0, True, tris)
+    o, d = ray_set(1500, seed=13)
+    res = sc.intersect(o, d, bound="box", stats=True, threads=1)
+    rec = sc.records(o, d, bound="box")
+    st = res["stats"]
+    assert len(rec) == st["node_tests"].sum() > 10000
+    assert (np.bincount(rec["ray"], minlength=len(o)) == st["node_tests"]).all()
+    per_level = np.zeros_like(st["node_tests_per_level"])
+    np.add.at(per_level, (rec["ray"], rec["level"]), 1)
+    assert (per_level == st["node_tests_per_level"]).all()
+    # Ray by ray in walk order, seq counting each ray's tests from 0.
+    assert (np.diff(rec["ray"]) >= 0).all()
+    starts = np.concatenate([[0], np.cumsum(st["node_tests"])[:-1]])
+    assert (rec["seq"] == np.arange(len(rec)) - np.repeat(starts, st["node_tests"])).all()
+    # The box's decision is the walk's.
+    same = (rec["ray"][1:] == rec["ray"][:-1]) & (rec["prim"][1:] == rec["prim"][:-1])
+    down = (rec["box_pass"][:-1] == 1) & (rec["level"][:-1] > 0)
+    assert same.sum() > 5000 and (down & same).sum() > 1000
+    assert (rec["level"][1:][down & same] == rec["level"][:-1][down & same] - 1).all()
+    assert (rec["level"][1:][~down & same] >= rec["level"][:-1][~down & same]).all()
+    # With t_max = inf a test passes whenever it passed with the walk's t_max.
+    assert (rec["box_length"][rec["box_pass"] == 1] >= 0).all()
+    assert (rec["slab_length"][rec["slab_pass"] == 1] >= 0).all()
+    both = rec["both_length"] >= 0
+    assert both.sum() > 1000
+    assert (rec["box_length"][both] >= 0).all() and (rec["slab_length"][both] >= 0).all()
+    assert (rec["both_length"][both] <= np.minimum(rec["box_length"], rec["slab_length"])[both] * (1 + 1e-9) + 1e-9).all()
+    # The slab shortens some intervals and prunes some nodes the box keeps.
+    assert ((rec["box_pass"] == 1) & (rec["slab_pass"] == 0)).sum() > 0
+    assert (rec["slab_length"][both] < rec["box_length"][both]).mean() > 0.1
+    # The box length against an independent clip of the node's box.
+    rng = np.random.default_rng(2)
+    for q in rng.choice(len(rec), size=400, replace=False):
+        r = rec[q]
+        q0, M = sc.tangent_frame(int(r["prim"]))
+        ot, dt = M @ (o[r["ray"]] - q0), M @ d[r["ray"]]
+        lo, hi = sc.node_bound(int(r["prim"]), int(r["level"]), int(r["i"]), int(r["j"]))["box"]
+        want = _box_clip_length(lo, hi, ot, dt, 0.0)
+        if want < 0:
+            assert r["box_length"] < 0
+        else:
+            assert abs(r["box_length"] - want) <= 1e-9 * (1 + want), (r, want)
