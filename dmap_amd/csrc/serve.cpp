@@ -15,6 +15,12 @@
 //    so that it is next;
 //  * chunking: a step holds at most step_rays rays; a larger request is
 //    traced over several steps, possibly on several slots at once;
+//  * backlog steps: while the rays waiting would fill every slot's next
+//    BACKLOG_STEPS steps of BACKLOG_FACTOR x step_rays, a step holds up to
+//    BACKLOG_FACTOR x step_rays. Each step's start, drain and finalize cost
+//    about the same at any size, so fewer, larger steps raise the rate under
+//    a backlog; at a light load the steps stay at step_rays, and a small
+//    request that arrives finds the slots running steps that short;
 //  * one kind per step: objects with constant normals and objects with
 //    interpolated normals use different kernel specialisations, so a step
 //    takes requests of the kind of its first request only.
@@ -100,6 +106,8 @@ struct Arena
 
 // Requests waiting longer than this go first, in arrival order.
 constexpr double AGING_S = 0.02;
+constexpr int64_t BACKLOG_FACTOR = 4;
+constexpr int64_t BACKLOG_STEPS = 1;
 // Hit pool bytes per ray: 8 fp32 + 3 int32 hit fields.
 constexpr size_t POOL_BYTES_PER_RAY = 44;
 
@@ -284,10 +292,16 @@ int pump(dmap_server *S)
             StepDesc D;
             D.pad_ = 0;
             const int kind = order.front()->obj->const_normal ? 1 : 0;
+            int64_t waiting = 0;
+            for (const Request *r : order)
+                waiting += r->n - r->next;
+            const int64_t big = BACKLOG_FACTOR * S->step_rays;
+            const int64_t cap =
+                waiting >= BACKLOG_STEPS * (int64_t)S->slots.size() * big && big <= MAX_STEP_RAYS ? big : S->step_rays;
             int64_t total = 0;
             int n_seg = 0;
             for (Request *r : order) {
-                if (n_seg == MAX_SEGMENTS || total >= S->step_rays)
+                if (n_seg == MAX_SEGMENTS || total >= cap)
                     break;
                 if ((r->obj->const_normal ? 1 : 0) != kind)
                     continue;
@@ -304,7 +318,7 @@ int pump(dmap_server *S)
                     r->bytes = bytes;
                     r->hits = hits_in(S->arena.base + off, r->n);
                 }
-                int64_t take = std::min(r->n - r->next, S->step_rays - total);
+                int64_t take = std::min(r->n - r->next, cap - total);
                 Segment &g = D.seg[n_seg++];
                 g.start = total;
                 g.first = r->next;

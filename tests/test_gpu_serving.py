@@ -66,6 +66,23 @@ def test_mix_equals_alone_and_plain(small_scenes):
     assert 0.2 < hit.mean() < 1.0
 
 
+def test_backlog_steps_equal_plain(small_scenes):
+    """60 requests of 2,000-12,000 rays on 2 slots of 4,096-ray steps: the
+    waiting rays (over four times slots x 4 x 4,096, the backlog
+    threshold) make the first steps hold up to 16,384 rays, and the later ones,
+    once the backlog has drained, 4,096; every request's hits equal the
+    single-request entry point's bit for bit."""
+    import dmap_amd
+    scn = small_scenes
+    reqs = serving.make_requests(scn, 60, seed=31, min_rays=2000, max_rays=12000)
+    jobs = [(scn[k].obj, r) for k, r in reqs]
+    assert sum(r.shape[1] for _, r in jobs) > 2 * 4 * 4096 * 4
+    srv = dmap_amd.Server(slots=2, step_rays=4096, hit_pool_bytes=64 << 20)
+    mix = _run(srv, jobs)
+    for q, (obj, rays) in enumerate(jobs):
+        assert _same(mix[q], _host(obj.intersect(rays))), f"request {q}"
+
+
 def test_memory_admission(small_scenes):
     """A hit pool that holds ~3 requests at once: the rest wait for releases."""
     import dmap_amd
