@@ -166,7 +166,10 @@ class TraversalOptions:
     slab_max_level: int = 1
     leaf: str = "newton"
     precision: str = "fp32"
-    reorder: bool = True
+    # Sort the rays by direction octant and origin before tracing: pays when
+    # rays of different kinds arrive interleaved; rays in generation order
+    # (pixel rows, surface points in turn) trace faster as given.
+    reorder: bool = False
     polish: bool = True  # False: report the search's own root, without the double polish
     # "shell": a constant-normal triangle's walk starts at the finest block of
     # cells holding the ray's clip to its height shell (the same hits, fewer

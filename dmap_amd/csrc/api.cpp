@@ -910,7 +910,7 @@ void dmap_object_destroy(dmap_object *obj)
 
 static int check_options(const dmap_options *opt, dmap_options &o)
 {
-    o = dmap_options{2, 1, 0, 0, 1, 1, 0};
+    o = dmap_options{2, 1, 0, 0, 0, 1, 0};
     if (opt)
         o = *opt;
     if (o.bound < 0 || o.bound > 2 || o.leaf < 0 || o.leaf > 1 || o.precision < 0 || o.precision > 1 ||

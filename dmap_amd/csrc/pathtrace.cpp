@@ -414,7 +414,7 @@ int dmap_pathtrace(const dmap_object *obj, const double *cam_o, const double *ca
     // compacted wave by wave in that order), which is coherent already:
     // sorting each batch cost more than it saved (20.4 against 17.2 ms for
     // 1024^2 x 4 spp on the 4096^2 plane), so the calls below do not.
-    dmap_options opt{2, 1, 0, 0, 0};
+    dmap_options opt{2, 1, 0, 0, 0, 1, 0};
     if (options)
         opt = *options;
     opt.reorder = 0;

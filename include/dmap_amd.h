@@ -63,8 +63,10 @@ typedef struct
     int slab_max_level; /* the slab is read at levels <= this (default 1) */
     int leaf;           /* 0 certified Newton (default), 1 two-triangle */
     int precision;      /* 0 fp32 (default), 1 the same solve in fp64 on the device */
-    int reorder;        /* 1 (default): sort the rays by direction octant and origin before
-                           tracing, for wave coherence; results are per ray either way */
+    int reorder;        /* 0 (default): trace the rays in their given order; 1: sort them by
+                           direction octant and origin first, which pays when rays of
+                           different kinds arrive interleaved (shuffled); results are per
+                           ray either way */
     int polish;         /* 1 (default): the reported root is polished by two Newton steps in
                            double; 0 reports the search's own root (precision's arithmetic),
                            evaluated in double, to measure the fp32 solve's error */

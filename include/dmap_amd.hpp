@@ -51,7 +51,7 @@ struct TraversalOptions
     int slab_max_level = 1;
     LeafMode leaf = LeafMode::Newton;
     bool fp64 = false;   // the same solve in double on the device
-    bool reorder = true; // sort rays for wave coherence (results are per ray either way)
+    bool reorder = false; // sort rays by octant and origin first (pays for interleaved kinds of rays)
     bool polish = true;  // polish the reported root in double (false: the search's own root)
     bool from_roots = false; // every walk starts at the triangle's roots, as the reference's does
 

@@ -39,7 +39,8 @@ def _call(dmap_amd, torch, obj, rays, entry, stats=False, stream=None):
     if stats:
         S = torch.zeros((len(dmap_amd.STAT_NAMES) + dmap_amd.STAT_LEVELS, n), dtype=torch.int32, device="cuda")
         st = dmap_amd._Stats(*[S[q].data_ptr() for q in range(len(dmap_amd.STAT_NAMES) + 1)])
-    opt = dmap_amd.TraversalOptions()._c()
+    # With the ray sort, whose buffers are most of the scratch.
+    opt = dmap_amd.TraversalOptions(reorder=True)._c()
     fn = getattr(dmap_amd._lib, entry)
     rc = fn(obj._h, ctypes.byref(r), n, ctypes.byref(h), -1, ctypes.byref(opt),
             ctypes.byref(st) if st is not None else None, dmap_amd._stream_ptr(stream))
