@@ -720,7 +720,6 @@ static void object_free(dmap_object *o)
     (void)hipFree(o->tris);
     (void)hipFree(o->boxes);
     (void)hipFree(o->bvh);
-    (void)hipFree(o->view_dev);
     delete o;
 }
 
@@ -792,9 +791,7 @@ static int object_bounds(dmap_object *o, const std::vector<HostTri> &tris)
     o->view.bvh = o->bvh;
     o->view.bvh_leaf = (const int *)(o->bvh + P);
     o->view.bvh_leaves = P;
-    e = hipMemcpy(o->view_dev, &o->view, sizeof(ObjectDev), hipMemcpyHostToDevice);
-    if (e == hipSuccess)
-        e = hipMemcpy(o->bvh, o->bvh_host.data(), sizeof(BvhNode) * P, hipMemcpyHostToDevice);
+    e = hipMemcpy(o->bvh, o->bvh_host.data(), sizeof(BvhNode) * P, hipMemcpyHostToDevice);
     if (e == hipSuccess)
         e = hipMemcpy(o->bvh + P, o->bvh_leaf_host.data(), sizeof(int) * P, hipMemcpyHostToDevice);
     if (e == hipSuccess)
@@ -849,8 +846,6 @@ int dmap_object_create(dmap_ctx *ctx, dmap_tile *tile, int n_tri, const double *
     o->uvs.assign(uvs, uvs + 6 * (size_t)n_tri);
     o->bvh_leaves = bvh_leaves_for(n_tri);
     e = hipMalloc(&o->bvh, (sizeof(BvhNode) + sizeof(int)) * (size_t)o->bvh_leaves);
-    if (e == hipSuccess)
-        e = hipMalloc(&o->view_dev, sizeof(ObjectDev));
     if (e != hipSuccess) {
         object_free(o);
         return fail(DMAP_ERR_MEMORY, "dmap_object_create: %s", hipGetErrorString(e));

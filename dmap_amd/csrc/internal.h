@@ -50,8 +50,7 @@ struct dmap_object
     TriRecord *tris = nullptr;
     float *boxes = nullptr;
     BvhNode *bvh = nullptr; // bvh_leaves nodes, then bvh_leaves leaf entries (int)
-    ObjectDev view{};              // what a single-object launch passes
-    ObjectDev *view_dev = nullptr; // the same, in device memory, for multi-request launches
+    ObjectDev view{}; // what a launch passes in its arguments
     std::vector<double> positions, normals, uvs; // the input triangles, for dmap_object_refresh
 };
 

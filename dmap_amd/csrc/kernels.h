@@ -99,13 +99,12 @@ struct ObjectDev
 struct Segment
 {
     int64_t start, first;
-    const ObjectDev *obj;
     const float *rays;
     int64_t rays_stride;
     float *hits;
     int64_t hits_n;
 };
-static_assert(sizeof(Segment) == 56, "a step segment is 56 bytes");
+static_assert(sizeof(Segment) == 48, "a step segment is 48 bytes");
 __host__ __device__ inline RaysDev segment_rays(const Segment &g)
 {
     if (g.rays_stride == 0)
@@ -122,14 +121,29 @@ __host__ __device__ inline HitsDev segment_hits(const Segment &g)
     return HitsDev{f, f + n, f + 2 * n, f + 3 * n, f + 4 * n, f + 5 * n, f + 6 * n, f + 7 * n, i, i + n, i + 2 * n};
 }
 // A step's descriptor travels in the kernels' arguments (no upload before
-// the launch): at most MAX_SEGMENTS requests per step keep it under 3 KB.
-constexpr int MAX_SEGMENTS = 48;
+// the launch); with at most MAX_SEGMENTS requests and MAX_GROUPS objects per
+// step it stays, with TraceParams, under the 4 KB of kernel arguments.
+constexpr int MAX_SEGMENTS = 32;
+constexpr int MAX_GROUPS = 8;
 constexpr int64_t MAX_STEP_RAYS = (int64_t)1 << 30;
+// One object's share of a serving step: the requests seg[seg_lo, seg_hi),
+// whose rays fill the step slots [start, end), traced by the trace kernel's
+// workgroups [wg_lo, wg_hi) and finalized by the finalize kernel's
+// workgroups [fwg_lo, fwg_hi). A workgroup serves one object only, so the
+// object's fields are uniform across its waves and its BVH fits in LDS.
+struct StepGroup
+{
+    ObjectDev obj;
+    int64_t start, end;
+    int wg_lo, wg_hi, fwg_lo, fwg_hi;
+    int seg_lo, seg_hi;
+};
 struct StepDesc
 {
     int64_t total;
-    int n_seg, pad_;
+    int n_seg, n_grp;
     Segment seg[MAX_SEGMENTS];
+    StepGroup grp[MAX_GROUPS];
 };
 
 // A closest-hit trace's result for one ray, one aligned 32-byte sector:
@@ -156,13 +170,16 @@ struct TraceParams
     int any_hit;
     int64_t n_rays;
     const uint32_t *perm; // lane order of the rays (NULL: as given)
-    unsigned *queue;      // the launch's ray queue head (reset by launch_trace)
+    unsigned *queue;      // the launch's ray queue head (reset by launch_trace); a serving step's
+                          // group g takes queue[g]
     RaysDev rays;
     HitsDev hits;
     RawHit *raw; // n_rays records for k_finalize (NULL: the raw record goes into the hit arrays)
     uint8_t *occluded;
     StatsDev stats;
 };
+
+static_assert(sizeof(TraceParams) + sizeof(StepDesc) <= 4096, "a step's kernel arguments exceed 4 KB");
 
 // Sort rays for coherence (reorder.hip): writes a permutation of 0..n-1
 // into the workspace and returns it in *perm.
@@ -217,6 +234,10 @@ struct StepGraph
     hipGraphNode_t trace = nullptr, finalize = nullptr;
     bool const_normal = false;
 };
+// The trace and finalize workgroups of a step, shared out among its groups
+// in proportion to their rays (at least one each): fills the groups'
+// wg_lo/wg_hi and fwg_lo/fwg_hi from their start/end.
+void step_partition(StepDesc &d);
 hipError_t step_graph_create(unsigned *queue, bool const_normal, StepGraph &g);
 hipError_t step_graph_launch(StepGraph &g, const TraceParams &p, const StepDesc &desc, hipStream_t stream);
 void step_graph_destroy(StepGraph &g);

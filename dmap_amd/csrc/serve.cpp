@@ -23,7 +23,18 @@
 //    request that arrives finds the slots running steps that short;
 //  * one kind per step: objects with constant normals and objects with
 //    interpolated normals use different kernel specialisations, so a step
-//    takes requests of the kind of its first request only.
+//    takes requests of the kind of its first request only;
+//  * object groups: a step holds requests of up to MAX_GROUPS objects, its
+//    slots grouped by object, and the step kernels' workgroups are shared
+//    out among the groups in proportion to their rays. A workgroup serves
+//    one object only, reading it from the kernel arguments (uniform across
+//    its waves, in scalar registers) with the object's BVH in LDS, as a
+//    single-object launch does. (With the objects mixed inside waves each
+//    lane read its object's fields through its own pointer and the BVH from
+//    global memory: the step kernel was 24% slower per ray than the
+//    single-object launch on the same rays. Steps of one object each were
+//    as fast per ray but needed a step per object, and small requests of
+//    different objects waited for steps in turn.)
 // A step is one persistent multi-request launch (reset, trace, finalize)
 // built once per slot and kind as a HIP graph and replayed with the step's
 // descriptor in the kernels' arguments (no upload kernel before it), so
@@ -290,7 +301,6 @@ int pump(dmap_server *S)
                 return a->seq < c->seq;
             });
             StepDesc D;
-            D.pad_ = 0;
             const int kind = order.front()->obj->const_normal ? 1 : 0;
             int64_t waiting = 0;
             for (const Request *r : order)
@@ -298,13 +308,27 @@ int pump(dmap_server *S)
             const int64_t big = BACKLOG_FACTOR * S->step_rays;
             const int64_t cap =
                 waiting >= BACKLOG_STEPS * (int64_t)S->slots.size() * big && big <= MAX_STEP_RAYS ? big : S->step_rays;
+            // The step's requests in order, each with its object's group.
+            struct Pick
+            {
+                Request *r;
+                int grp;
+                int64_t first, take;
+            };
+            std::vector<Pick> picks;
+            const dmap_object *objs[MAX_GROUPS];
+            int n_grp = 0;
             int64_t total = 0;
-            int n_seg = 0;
             for (Request *r : order) {
-                if (n_seg == MAX_SEGMENTS || total >= cap)
+                if ((int)picks.size() == MAX_SEGMENTS || total >= cap)
                     break;
                 if ((r->obj->const_normal ? 1 : 0) != kind)
                     continue;
+                int grp = 0;
+                while (grp < n_grp && objs[grp] != r->obj)
+                    ++grp;
+                if (grp == MAX_GROUPS)
+                    continue; // a ninth object waits for a later step
                 if (!r->admitted) {
                     size_t bytes = request_bytes(r->n);
                     size_t off;
@@ -318,27 +342,48 @@ int pump(dmap_server *S)
                     r->bytes = bytes;
                     r->hits = hits_in(S->arena.base + off, r->n);
                 }
+                if (grp == n_grp)
+                    objs[n_grp++] = r->obj;
                 int64_t take = std::min(r->n - r->next, cap - total);
-                Segment &g = D.seg[n_seg++];
-                g.start = total;
-                g.first = r->next;
-                g.obj = r->obj->view_dev;
-                g.rays = r->rays_base;
-                g.rays_stride = r->rays_stride;
-                g.hits = r->hits.t;
-                g.hits_n = r->n;
+                picks.push_back(Pick{r, grp, r->next, take});
                 r->next += take;
                 ++r->inflight;
                 total += take;
                 b->reqs.push_back(r);
             }
-            if (n_seg == 0)
+            if (picks.empty())
                 return DMAP_OK; // nothing fits the hit pool yet
             S->pending.erase(std::remove_if(S->pending.begin(), S->pending.end(),
                                             [](const Request *r) { return r->next == r->n; }),
                              S->pending.end());
+            // The step slots group by object: group g's requests fill
+            // [grp[g].start, grp[g].end), in the order they were picked.
+            int n_seg = 0;
+            int64_t at = 0;
+            for (int g = 0; g < n_grp; ++g) {
+                StepGroup &G = D.grp[g];
+                G.obj = objs[g]->view;
+                G.start = at;
+                G.seg_lo = n_seg;
+                for (const Pick &k : picks) {
+                    if (k.grp != g)
+                        continue;
+                    Segment &sg = D.seg[n_seg++];
+                    sg.start = at;
+                    sg.first = k.first;
+                    sg.rays = k.r->rays_base;
+                    sg.rays_stride = k.r->rays_stride;
+                    sg.hits = k.r->hits.t;
+                    sg.hits_n = k.r->n;
+                    at += k.take;
+                }
+                G.end = at;
+                G.seg_hi = n_seg;
+            }
             D.total = total;
             D.n_seg = n_seg;
+            D.n_grp = n_grp;
+            step_partition(D);
             TraceParams p;
             std::memset(&p, 0, sizeof p);
             p.queue = b->queue;

@@ -26,6 +26,7 @@
 // in texel-local coordinates (a, b) in [0, 1]^2, where its terms are small.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <atomic>
 #include <cstring>
 
@@ -52,9 +53,9 @@ constexpr int KNEWTON = 16;    // ulps of the residual's magnitude accepted as c
 constexpr int SUBDIV_DEPTH = 5; // displaced_intersector.h:132
 constexpr int MAX_ITERS = 20;   // displaced_intersector.h:129
 constexpr int STACK = 16;       // 4 + 3 x 4 entries for a depth-5 search
-// BVHs of at most this many leaves go to LDS in single-object launches
-// (64 B per node + 4 B per leaf).
+// BVHs of at most this many leaves go to LDS (64 B per node + 4 B per leaf).
 constexpr int LDS_BVH_LEAVES = 128;
+constexpr size_t BVH_LDS_BYTES = (sizeof(BvhNode) + sizeof(int)) * LDS_BVH_LEAVES;
 constexpr int BLOCK = 256;
 
 template <class R> __device__ inline R pad_of(R magnitude) { return (R)KPAD * Num<R>::eps * magnitude; }
@@ -1439,20 +1440,34 @@ template <class R, bool STATS, bool CN, bool TWO, bool MULTI>
 __device__ __forceinline__ void trace_body(const TraceParams &p, const StepDesc *sd)
 {
 
+    // Multi-request launches: the workgroup's object group (uniform across
+    // the workgroup); workgroups past the groups' shares have no work.
+    int grp = 0;
+    if (MULTI) {
+        grp = -1;
+        for (int g = 0; g < sd->n_grp; ++g)
+            if ((int)blockIdx.x >= sd->grp[g].wg_lo && (int)blockIdx.x < sd->grp[g].wg_hi)
+                grp = g;
+        if (grp < 0)
+            return;
+    }
+    // The object: the launch's, or the workgroup's group's in a serving
+    // step, in the kernel arguments either way.
+    const ObjectDev &OBJ = MULTI ? sd->grp[grp].obj : p.obj;
+
     // The leaf search stacks: STACK entries per thread, stride blockDim.x
     // (trace_lds_bytes per launch).
     extern __shared__ __align__(16) unsigned char s_lds[];
     typename StackSlot<R>::T *s_t = (typename StackSlot<R>::T *)s_lds;
     uint16_t *s_code = (uint16_t *)(s_t + STACK * blockDim.x);
-    // Single-object launches over a small BVH keep a copy of it in LDS,
-    // after the stacks: the walk's dependent node reads are the bulk of the
-    // per-triangle bookkeeping.
-    const bool lds_bvh = !MULTI && p.obj.bvh_leaves <= LDS_BVH_LEAVES;
+    // A small BVH is copied into LDS after the stacks: the walk's dependent
+    // node reads are the bulk of the per-triangle bookkeeping.
+    const bool lds_bvh = OBJ.bvh_leaves <= LDS_BVH_LEAVES;
     unsigned char *bvh_lds = (unsigned char *)(s_code + STACK * blockDim.x);
     // Multi-request launches: the step's segments, copied from the kernel
-    // arguments into LDS (after the stacks) for the refill's binary search
-    // and the lanes' request records.
-    Segment *s_seg = (Segment *)bvh_lds;
+    // arguments into LDS (after the BVH's room) for the refill's binary
+    // search and the lanes' request records.
+    Segment *s_seg = (Segment *)(bvh_lds + BVH_LDS_BYTES);
     if (MULTI) {
         const uint32_t *src = (const uint32_t *)sd->seg;
         uint32_t *dst = (uint32_t *)s_seg;
@@ -1463,13 +1478,13 @@ __device__ __forceinline__ void trace_body(const TraceParams &p, const StepDesc 
     }
     if (lds_bvh) {
         uint32_t *dst = (uint32_t *)bvh_lds;
-        const uint32_t *src = (const uint32_t *)p.obj.bvh;
-        const int node_words = p.obj.bvh_leaves * (int)(sizeof(BvhNode) / 4);
+        const uint32_t *src = (const uint32_t *)OBJ.bvh;
+        const int node_words = OBJ.bvh_leaves * (int)(sizeof(BvhNode) / 4);
         for (int q = threadIdx.x; q < node_words; q += blockDim.x)
             dst[q] = src[q];
-        const uint32_t *lsrc = (const uint32_t *)p.obj.bvh_leaf;
+        const uint32_t *lsrc = (const uint32_t *)OBJ.bvh_leaf;
         uint32_t *ldst = (uint32_t *)(bvh_lds + sizeof(BvhNode) * LDS_BVH_LEAVES);
-        for (int q = threadIdx.x; q < p.obj.bvh_leaves; q += blockDim.x)
+        for (int q = threadIdx.x; q < OBJ.bvh_leaves; q += blockDim.x)
             ldst[q] = lsrc[q];
         __syncthreads();
     }
@@ -1477,15 +1492,16 @@ __device__ __forceinline__ void trace_body(const TraceParams &p, const StepDesc 
     const int lane = threadIdx.x & 63;
     // Persistent waves: a lane that has finished its ray writes the result
     // and, once enough lanes of its wave are idle, the wave claims that many
-    // new rays from the launch's queue; the wave exits when the queue is
-    // drained and its lanes are done.
+    // new rays from the launch's queue (its group's, in a serving step); the
+    // wave exits when the queue is drained and its lanes are done.
     bool drained = false, have = false;
     int64_t rid = 0;
-    // Multi-request launches: the lane's request and its object.
+    // Multi-request launches: the lane's request.
     const Segment *SG = nullptr;
-    const ObjectDev *O = nullptr;
-    const int64_t n_total = MULTI ? sd->total : p.n_rays;
-#define DMAP_OBJ (MULTI ? *O : p.obj)
+    const int64_t n_total = MULTI ? sd->grp[grp].end - sd->grp[grp].start : p.n_rays;
+    const int64_t slot0 = MULTI ? sd->grp[grp].start : 0;
+    unsigned *const queue = MULTI ? p.queue + grp : p.queue;
+#define DMAP_OBJ (OBJ)
 #define DMAP_RAYS (MULTI ? segment_rays(*SG) : p.rays)
 #define DMAP_HITS (MULTI ? segment_hits(*SG) : p.hits)
     R t_min = 0, t_best = 0;
@@ -1500,7 +1516,7 @@ __device__ __forceinline__ void trace_body(const TraceParams &p, const StepDesc 
     R hit_a = 0, hit_b = 0;
     LaneStats st{};
     Walk<R> wk;
-    const TriRecord *T = p.obj.tris;
+    const TriRecord *T = OBJ.tris;
     int root_level = 0, rb_i = 0, rb_j = 0;
     int lvl = 0, pi = 0, pj = 0; // the group being walked: the cells at lvl below (pi, pj), or the root block
     uint64_t trail = 0;
@@ -1562,7 +1578,7 @@ __device__ __forceinline__ void trace_body(const TraceParams &p, const StepDesc 
                 int leader = __ffsll((unsigned long long)idle) - 1;
                 unsigned base = 0;
                 if (lane == leader)
-                    base = atomicAdd(p.queue, (unsigned)cnt);
+                    base = atomicAdd(queue, (unsigned)cnt);
                 base = __shfl(base, leader);
                 if ((int64_t)base + cnt >= n_total)
                     drained = true;
@@ -1571,8 +1587,10 @@ __device__ __forceinline__ void trace_body(const TraceParams &p, const StepDesc 
                     if (slot < n_total) {
                         have = true;
                         if (MULTI) {
-                            // The request holding this slot: a binary search of the starts.
-                            int lo = 0, hi = sd->n_seg - 1;
+                            // The request holding this step slot: a binary
+                            // search of the group's segments' starts.
+                            slot += slot0;
+                            int lo = sd->grp[grp].seg_lo, hi = sd->grp[grp].seg_hi - 1;
                             while (lo < hi) {
                                 int mid = (lo + hi + 1) >> 1;
                                 if (s_seg[mid].start <= slot)
@@ -1581,7 +1599,6 @@ __device__ __forceinline__ void trace_body(const TraceParams &p, const StepDesc 
                                     hi = mid - 1;
                             }
                             SG = &s_seg[lo];
-                            O = SG->obj;
                             rid = slot - SG->start + SG->first;
                         } else {
                             rid = p.perm ? (int64_t)p.perm[slot] : slot;
@@ -1670,9 +1687,9 @@ __device__ __forceinline__ void trace_body(const TraceParams &p, const StepDesc 
                         k = p.prim_mask;
                 }
             } else {
-                if (!MULTI && lds_bvh)
+                if (lds_bvh)
                     k = bvh_next<R>((const BvhNode *)bvh_lds, (const int *)(bvh_lds + sizeof(BvhNode) * LDS_BVH_LEAVES),
-                                    (uint32_t)p.obj.bvh_leaves, bvh_cur, bvh_trail, o_f, d_f, inv_f, t_min, t_best,
+                                    (uint32_t)OBJ.bvh_leaves, bvh_cur, bvh_trail, o_f, d_f, inv_f, t_min, t_best,
                                     hit_tri >= 0);
                 else
                     k = bvh_next<R>(DMAP_OBJ.bvh, DMAP_OBJ.bvh_leaf, (uint32_t)DMAP_OBJ.bvh_leaves, bvh_cur,
@@ -1949,14 +1966,22 @@ __global__ void __launch_bounds__(256) k_finalize(TraceParams p)
     }
 }
 
-// The same for a serving step: grid-stride over the step's slots (the grid
-// is fixed for graph replay), the segments staged in LDS for the search. It
+// The same for a serving step: each workgroup finalizes its object group's
+// step slots, grid-strided over the group's workgroups (the grid is fixed
+// for graph replay), with the segments staged in LDS for the search. It
 // runs while other slots' persistent traces hold most of the CUs, so its
 // waves are kept small (one-wave workgroups, at most 168 VGPRs for 3 waves
 // per SIMD): 256-thread workgroups at 182 VGPRs took twice as long there.
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 8)))
 k_finalize_multi(TraceParams p, StepDesc sd)
 {
+    int grp = -1;
+    for (int g = 0; g < sd.n_grp; ++g)
+        if ((int)blockIdx.x >= sd.grp[g].fwg_lo && (int)blockIdx.x < sd.grp[g].fwg_hi)
+            grp = g;
+    if (grp < 0)
+        return;
+    const StepGroup &GR = sd.grp[grp];
     __shared__ Segment s_seg[MAX_SEGMENTS];
     {
         const uint32_t *src = (const uint32_t *)sd.seg;
@@ -1966,9 +1991,10 @@ k_finalize_multi(TraceParams p, StepDesc sd)
             dst[q] = src[q];
     }
     __syncthreads();
-    for (int64_t slot = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; slot < sd.total;
-         slot += (int64_t)gridDim.x * blockDim.x) {
-        int lo = 0, hi = sd.n_seg - 1;
+    const int64_t stride = (int64_t)(GR.fwg_hi - GR.fwg_lo) * blockDim.x;
+    for (int64_t slot = GR.start + (int64_t)(blockIdx.x - GR.fwg_lo) * blockDim.x + threadIdx.x; slot < GR.end;
+         slot += stride) {
+        int lo = GR.seg_lo, hi = GR.seg_hi - 1;
         while (lo < hi) {
             int mid = (lo + hi + 1) >> 1;
             if (s_seg[mid].start <= slot)
@@ -1977,8 +2003,7 @@ k_finalize_multi(TraceParams p, StepDesc sd)
                 hi = mid - 1;
         }
         const Segment &G = s_seg[lo];
-        const ObjectDev *obj = G.obj;
-        finalize_one(p, obj->pyr, obj->tris, segment_rays(G), segment_hits(G), slot - G.start + G.first);
+        finalize_one(p, GR.obj.pyr, GR.obj.tris, segment_rays(G), segment_hits(G), slot - G.start + G.first);
     }
 }
 
@@ -2388,27 +2413,31 @@ hipError_t launch_height_range(const PyramidView &P, const TriRecord *tris, int 
     return hipGetLastError();
 }
 
-__global__ void k_queue_reset(unsigned *queue) { *queue = 0u; }
+// Zeroes blockDim.x queue heads (one for a launch, MAX_GROUPS for a step).
+__global__ void k_queue_reset(unsigned *queue) { queue[threadIdx.x] = 0u; }
 
 template <class R> static size_t trace_lds_bytes(unsigned block)
 {
     return (size_t)STACK * block * (sizeof(typename StackSlot<R>::T) + sizeof(uint16_t));
 }
 
-// The LDS copy of a small BVH in single-object launches (k_trace's lds_bvh).
+// The LDS copy of a small BVH (trace_body's lds_bvh).
 template <class R> static size_t bvh_lds_bytes(const ObjectDev &obj)
 {
-    return obj.bvh_leaves <= LDS_BVH_LEAVES ? (sizeof(BvhNode) + sizeof(int)) * LDS_BVH_LEAVES : 0;
+    return obj.bvh_leaves <= LDS_BVH_LEAVES ? BVH_LDS_BYTES : 0;
 }
 
 // Persistent single-object launches: workgroups per CU when the runtime
 // cannot say how many stay resident.
 constexpr int TRACE_GROUPS_PER_CU = 3;
 
-// Multi-request launches use one-wave workgroups: a workgroup's resources
-// free as soon as its wave is done, so the next step's waves fill the tail
-// of a running step wave by wave.
-constexpr unsigned MULTI_BLOCK = 64;
+// Multi-request launches use two-wave workgroups: a workgroup's resources
+// free as soon as its waves are done, so the next step's waves fill the
+// tail of a running step almost wave by wave, while two waves share one LDS
+// copy of the object's BVH (one-wave workgroups with a copy each would hold
+// 2.25 instead of 3 waves per SIMD).
+constexpr unsigned MULTI_BLOCK = 128;
+constexpr unsigned MULTI_FINALIZE_BLOCK = 64;
 
 static int cu_count();
 
@@ -2504,12 +2533,39 @@ hipError_t launch_trace(const TraceParams &p, int precision, hipStream_t stream,
     return hipGetLastError();
 }
 
-// The step graph's launch shapes: as many trace waves as the persistent
-// single-object launch, in one-wave workgroups; a fixed finalize grid that
-// strides over the step.
-static unsigned multi_trace_blocks() { return (unsigned)cu_count() * 4 * (BLOCK / MULTI_BLOCK); }
-static size_t multi_trace_lds() { return trace_lds_bytes<float>(MULTI_BLOCK) + sizeof(Segment) * MAX_SEGMENTS; }
-constexpr unsigned MULTI_FINALIZE_BLOCKS = 4096;
+// The step graph's launch shapes: as many trace waves as stay resident with
+// the persistent single-object launch (3 per SIMD), in two-wave workgroups
+// with the stacks, the BVH's room and the segments in LDS; a finalize grid
+// of one-wave workgroups that strides over the step. Both grids are fixed
+// for graph replay and hold room for the one workgroup each group gets at
+// least.
+static unsigned multi_trace_share() { return (unsigned)cu_count() * 12 / (MULTI_BLOCK / 64); }
+static unsigned multi_trace_blocks() { return multi_trace_share() + MAX_GROUPS; }
+static size_t multi_trace_lds()
+{
+    return trace_lds_bytes<float>(MULTI_BLOCK) + BVH_LDS_BYTES + sizeof(Segment) * MAX_SEGMENTS;
+}
+constexpr unsigned MULTI_FINALIZE_SHARE = 4096;
+constexpr unsigned MULTI_FINALIZE_BLOCKS = MULTI_FINALIZE_SHARE + MAX_GROUPS;
+
+void step_partition(StepDesc &d)
+{
+    const unsigned share[2] = {multi_trace_share(), MULTI_FINALIZE_SHARE};
+    int next[2] = {0, 0};
+    for (int g = 0; g < d.n_grp; ++g) {
+        StepGroup &G = d.grp[g];
+        const int64_t n = G.end - G.start;
+        int cnt[2];
+        for (int k = 0; k < 2; ++k) {
+            int64_t c = d.total > 0 ? (int64_t)share[k] * n / d.total : 1;
+            cnt[k] = (int)std::max<int64_t>(1, c);
+        }
+        G.wg_lo = next[0];
+        G.wg_hi = next[0] += cnt[0];
+        G.fwg_lo = next[1];
+        G.fwg_hi = next[1] += cnt[1];
+    }
+}
 
 static void step_node_params(const StepGraph &g, bool trace, void **args, hipKernelNodeParams &kp)
 {
@@ -2525,7 +2581,7 @@ static void step_node_params(const StepGraph &g, bool trace, void **args, hipKer
     } else {
         kp.func = (void *)k_finalize_multi;
         kp.gridDim = dim3(MULTI_FINALIZE_BLOCKS);
-        kp.blockDim = dim3(MULTI_BLOCK);
+        kp.blockDim = dim3(MULTI_FINALIZE_BLOCK);
     }
     kp.kernelParams = args;
 }
@@ -2545,7 +2601,7 @@ hipError_t step_graph_create(unsigned *queue, bool const_normal, StepGraph &g)
     void *reset_args[1] = {&queue};
     kp.func = (void *)k_queue_reset;
     kp.gridDim = dim3(1);
-    kp.blockDim = dim3(1);
+    kp.blockDim = dim3(MAX_GROUPS);
     kp.kernelParams = reset_args;
     TraceParams p;
     std::memset(&p, 0, sizeof p);

@@ -248,11 +248,13 @@ int dmap_kernel_timing(int enable);
 int dmap_kernel_timing_read(double *trace_ms, int64_t *launches);
 
 /* Serving (continuous batching; no counterpart in the reference, which
- * traces one ray per call). Requests against resident objects are queued,
- * admitted in arrival order into the next free slot while the step's ray
- * budget and the hit pool allow, and traced by one persistent multi-request
- * launch per step (replayed from a captured HIP graph). Up to n_slots steps
- * overlap on the device. Ray buffers stay owned by the caller and must
+ * traces one ray per call). Requests against resident objects are queued
+ * and admitted (those waiting over 20 ms first, in arrival order, then by
+ * fewest rays left) into the next free slot while the step's ray budget
+ * (step_rays, 4 x step_rays under a backlog), its 32 requests, its 8
+ * objects and the hit pool allow. A step is one persistent multi-request
+ * launch (replayed from a captured HIP graph) whose workgroups are shared
+ * out among the step's objects. Up to n_slots steps overlap on the device. Ray buffers stay owned by the caller and must
  * stay valid until the request completes; hits live in the server's pool
  * (44 B per ray) until released. */
 typedef struct dmap_server dmap_server;

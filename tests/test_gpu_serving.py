@@ -154,7 +154,7 @@ def test_both_normal_kinds(small_scenes, torch_gpu):
 
 def test_full_steps_of_tiny_requests(small_scenes):
     """300 requests of 8-40 rays in 1M-ray steps: a step fills up to its
-    256-request limit (the step kernels stage the request starts in LDS and
+    32-request limit (the step kernels stage the request starts in LDS and
     search them per slot), and every request's hits equal the
     single-request entry point's."""
     import dmap_amd
@@ -230,7 +230,7 @@ def test_pool_not_a_multiple_of_256(small_scenes):
 def test_scattered_ray_planes_and_the_segment_cap(small_scenes):
     """Requests whose eight ray arrays are separate allocations (not one
     stride apart) go through the server's device record of their planes;
-    and 120 tiny requests fill steps up to their 48-request cap. Every
+    and 120 tiny requests fill steps up to their 32-request cap. Every
     request's hits equal the single-request call's."""
     import ctypes
 
@@ -259,3 +259,29 @@ def test_scattered_ray_planes_and_the_segment_cap(small_scenes):
     for k, t in enumerate(tiny):
         assert _same(got[1 + k], _host(scn.obj.intersect(t))), k
     assert sum(int(np.isfinite(got[1 + k]["t"]).sum()) for k in range(120)) > 0
+
+
+def test_more_objects_than_a_step_holds(small_scenes):
+    """44 requests against 11 objects, more than the 8 object groups a step
+    holds: quad tilings of 1 to 12 quads a side over the three tiles (the
+    12 x 12 tiling has 288 triangles, a BVH too large for LDS), requests of
+    500-5,000 rays, 16,384-ray steps on 2 slots. Steps hold several objects,
+    each traced by its own share of the workgroups; every request's hits
+    equal the single-request entry point's."""
+    import dmap_amd
+    scn = small_scenes
+    sides = (1, 2, 3, 4, 5, 6, 8, 9, 10, 11, 12)
+    objs = [dmap_amd.ObjectIntersector(scn[k % 3].tile, W.quad_plane(m, 8.0)) for k, m in enumerate(sides)]
+    rng = np.random.default_rng(41)
+    jobs = []
+    for q in range(44):
+        k = q % len(objs)
+        n = int(rng.integers(500, 5001))
+        pool = scn[k % 3].primary if q % 2 else scn[k % 3].bounce
+        start = int(rng.integers(0, pool.shape[1] - n))
+        jobs.append((objs[k], pool[:, start:start + n].contiguous()))
+    srv = dmap_amd.Server(slots=2, step_rays=16384, hit_pool_bytes=64 << 20)
+    mix = _run(srv, jobs)
+    for q, (obj, rays) in enumerate(jobs):
+        assert _same(mix[q], _host(obj.intersect(rays))), f"request {q}"
+    assert np.mean([np.isfinite(m["t"]).mean() for m in mix.values()]) > 0.2
