@@ -41,11 +41,14 @@ STAT_LEVELS = 16  # DMAP_STAT_LEVELS: node tests per level, levels 0 .. 15
 EXPORTS = ("dmap_last_error", "dmap_version", "dmap_ctx_create", "dmap_ctx_destroy", "dmap_device_alloc",
            "dmap_device_free", "dmap_memcpy", "dmap_stream_synchronize", "dmap_tile_create",
            "dmap_tile_rebuild", "dmap_tile_info", "dmap_tile_read_level", "dmap_tile_read_nodes",
-           "dmap_tile_destroy", "dmap_object_create", "dmap_object_triangle_info", "dmap_object_destroy",
-           "dmap_intersect_closest", "dmap_intersect_closest_noalloc", "dmap_intersect_any", "dmap_tile_set_texels", "dmap_object_refresh",
-           "dmap_intersect_adjoint", "dmap_hit_record", "dmap_kernel_timing", "dmap_kernel_timing_read", "dmap_server_create", "dmap_server_submit", "dmap_server_poll",
-           "dmap_server_hits", "dmap_server_copy_hits", "dmap_server_release", "dmap_server_idle",
-           "dmap_server_destroy", "dmap_pathtrace")
+           "dmap_tile_destroy", "dmap_object_create", "dmap_object_create_ex", "dmap_object_triangle_info",
+           "dmap_object_destroy", "dmap_intersect_closest", "dmap_intersect_closest_noalloc", "dmap_intersect_any",
+           "dmap_tile_set_texels", "dmap_object_refresh", "dmap_intersect_adjoint", "dmap_hit_record",
+           "dmap_kernel_timing", "dmap_kernel_timing_read", "dmap_server_create", "dmap_server_submit",
+           "dmap_server_poll", "dmap_server_hits", "dmap_server_copy_hits", "dmap_server_release",
+           "dmap_server_idle", "dmap_server_destroy", "dmap_pathtrace")
+# dmap_object_create_ex's BVH builders (DMAP_BVH_*).
+BVH_BUILDERS = {"auto": 0, "host": 1, "device": 2}
 
 
 class DmapError(RuntimeError):
@@ -119,6 +122,7 @@ def load_library():
         L.dmap_tile_read_nodes.argtypes = [vp, vp]
         L.dmap_tile_destroy.argtypes = [vp]
         L.dmap_object_create.argtypes = [vp, vp, c.c_int, vp, vp, vp, c.POINTER(vp)]
+        L.dmap_object_create_ex.argtypes = [vp, vp, c.c_int, vp, vp, vp, c.c_int, c.POINTER(vp)]
         L.dmap_object_triangle_info.argtypes = [vp, c.c_int, c.POINTER(c.c_int), c.POINTER(c.c_int), vp, vp, vp]
         L.dmap_object_destroy.argtypes = [vp]
         L.dmap_intersect_closest.argtypes = [vp, c.POINTER(_Rays), c.c_int64, c.POINTER(_Hits), c.c_int,
@@ -277,9 +281,11 @@ class DisplacementTile:
 
 class ObjectIntersector:
     """Base triangles over one tile (ObjectIntersector). triangles: (n, 24)
-    float64 rows p0 p1 p2 (9), vertex normals m0 m1 m2 (9), uvs t0 t1 t2 (6)."""
+    float64 rows p0 p1 p2 (9), vertex normals m0 m1 m2 (9), uvs t0 t1 t2 (6).
+    bvh: the top-level BVH's builder, "auto" (the device's from 16,384
+    triangles), "host" or "device"."""
 
-    def __init__(self, tile, triangles, ctx=None):
+    def __init__(self, tile, triangles, ctx=None, bvh="auto"):
         self.tile = tile
         self.ctx = ctx or tile.ctx
         tri = np.ascontiguousarray(triangles, dtype=np.float64).reshape(-1, 24)
@@ -288,9 +294,12 @@ class ObjectIntersector:
         nrm = np.ascontiguousarray(tri[:, 9:18])
         uvs = np.ascontiguousarray(tri[:, 18:24])
         self._h = ctypes.c_void_p()
-        _check(load_library().dmap_object_create(self.ctx._h, tile._h, self.n_tri, ctypes.c_void_p(pos.ctypes.data),
-                                                 ctypes.c_void_p(nrm.ctypes.data), ctypes.c_void_p(uvs.ctypes.data),
-                                                 ctypes.byref(self._h)), "dmap_object_create")
+        if bvh not in BVH_BUILDERS:
+            raise DmapError(f"bvh must be one of {sorted(BVH_BUILDERS)}")
+        _check(load_library().dmap_object_create_ex(self.ctx._h, tile._h, self.n_tri,
+                                                    ctypes.c_void_p(pos.ctypes.data), ctypes.c_void_p(nrm.ctypes.data),
+                                                    ctypes.c_void_p(uvs.ctypes.data), BVH_BUILDERS[bvh],
+                                                    ctypes.byref(self._h)), "dmap_object_create")
 
     def triangle_info(self, k):
         rl, nr = ctypes.c_int(), ctypes.c_int()

@@ -22,8 +22,8 @@ ARCH = os.environ.get("DMAP_AMD_ARCH", "gfx950")
 COMMON = ["-O3", "-fPIC", "-std=c++17", f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang",
           "-packed-fp32-ops", "-Wno-unused-command-line-argument", f"-I{CSRC}"]
 # The pyramid build keeps the reference's double arithmetic bit for bit.
-PER_FILE = {"pyramid.hip": ["-ffp-contract=off"], "traverse.hip": [], "reorder.hip": [], "api.cpp": ["-x", "hip"],
-            "serve.cpp": ["-x", "hip"], "pathtrace.cpp": ["-x", "hip", "-ffp-contract=off"]}
+PER_FILE = {"pyramid.hip": ["-ffp-contract=off"], "traverse.hip": [], "reorder.hip": [], "bvh.hip": [],
+            "api.cpp": ["-x", "hip"], "serve.cpp": ["-x", "hip"], "pathtrace.cpp": ["-x", "hip", "-ffp-contract=off"]}
 
 
 def hipcc():

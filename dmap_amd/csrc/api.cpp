@@ -12,6 +12,7 @@
 #include <cstring>
 #include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "internal.h"
@@ -441,6 +442,36 @@ int bvh_leaves_for(int n_tri)
 static_assert(sizeof(dmap_node_record) == sizeof(NodeRecordDev) && sizeof(NodeRecordDev) == 48,
               "dmap_node_record and NodeRecordDev share one layout");
 
+namespace
+{
+
+// Runs body(k) for k in [0, n) on up to 16 host threads (chunks of
+// consecutive k); the per-triangle set-up of large meshes.
+template <class F> void parallel_for(int n, F body)
+{
+    const int hw = (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    const int threads = n < 4096 ? 1 : hw;
+    if (threads == 1) {
+        for (int k = 0; k < n; ++k)
+            body(k);
+        return;
+    }
+    std::vector<std::thread> pool;
+    const int chunk = (n + threads - 1) / threads;
+    for (int t = 0; t < threads; ++t) {
+        const int lo = t * chunk, hi = std::min(n, lo + chunk);
+        if (lo < hi)
+            pool.emplace_back([=] {
+                for (int k = lo; k < hi; ++k)
+                    body(k);
+            });
+    }
+    for (auto &th : pool)
+        th.join();
+}
+
+} // namespace
+
 extern "C" {
 
 const char *dmap_last_error(void) { return g_error.c_str(); }
@@ -723,6 +754,10 @@ static void object_free(dmap_object *o)
     delete o;
 }
 
+// Objects of at least this many triangles get the device BVH builder when
+// the caller leaves the choice to the library (DMAP_BVH_AUTO).
+constexpr int DEVICE_BVH_MIN_TRIANGLES = 1 << 14;
+
 // The triangles' height ranges over the tile's current pyramid, their
 // boxes, the top-level BVH and the object's device view (triangle_height_range,
 // triangle_aabb, node_bounds.cpp:218-269; ObjectIntersector::build_bvh,
@@ -745,10 +780,11 @@ static int object_bounds(dmap_object *o, const std::vector<HostTri> &tris)
     if (e != hipSuccess)
         return fail(DMAP_ERR_HIP, "object bounds: %s", hipGetErrorString(e));
     o->boxes_host.resize(6 * (size_t)n_tri);
-    for (int k = 0; k < n_tri; ++k) {
-        float lo = o->h_range_host[2 * k], hi = o->h_range_host[2 * k + 1];
-        if (!(lo <= hi))
+    for (int k = 0; k < n_tri; ++k)
+        if (!(o->h_range_host[2 * k] <= o->h_range_host[2 * k + 1]))
             return fail(DMAP_ERR_INVALID, "object bounds: triangle %d's footprint holds no cell", k);
+    parallel_for(n_tri, [&](int k) {
+        float lo = o->h_range_host[2 * k], hi = o->h_range_host[2 * k + 1];
         triangle_box(tris[k], lo, hi, &o->boxes_host[6 * k]);
         o->host[k].h_lo = lo;
         o->host[k].h_hi = hi;
@@ -756,33 +792,59 @@ static int object_bounds(dmap_object *o, const std::vector<HostTri> &tris)
             o->host[k].aabb_lo[q] = o->boxes_host[6 * k + q];
             o->host[k].aabb_hi[q] = o->boxes_host[6 * k + 3 + q];
         }
-    }
-    std::vector<int> idx(n_tri);
-    for (int k = 0; k < n_tri; ++k)
-        idx[k] = k;
+    });
+    e = hipMemcpy(o->boxes, o->boxes_host.data(), sizeof(float) * 6 * n_tri, hipMemcpyHostToDevice);
+    if (e != hipSuccess)
+        return fail(DMAP_ERR_HIP, "object bounds: %s", hipGetErrorString(e));
     const int P = o->bvh_leaves;
-    std::vector<BvhBuildNode> build((size_t)P, BvhBuildNode{});
-    o->bvh_leaf_host.assign((size_t)P, -1);
-    BvhBuildNode &root = build[0];
-    build_bvh_node(build, o->bvh_leaf_host, o->boxes_host, idx, 1, 0, n_tri, (uint32_t)P, root.lo[0], root.hi[0]);
-    root.empty = 2;
-    o->bvh_host.assign((size_t)P, BvhNode{});
-    for (int i = 0; i < P; ++i) {
-        const BvhBuildNode &b = build[(size_t)i];
-        BvhNode &nd = o->bvh_host[(size_t)i];
-        nd.axis = b.axis;
-        nd.empty = b.empty;
-        for (int c = 0; c < 2; ++c)
-            if (!(b.empty & (1 << c)))
-                bvh_centre_radius(b.lo[c], b.hi[c], nd.ctr[c], nd.rad[c]);
+    const bool on_device = o->bvh_builder == DMAP_BVH_DEVICE ||
+                           (o->bvh_builder == DMAP_BVH_AUTO && n_tri >= DEVICE_BVH_MIN_TRIANGLES);
+    if (on_device) {
+        const size_t ws_bytes = bvh_build_workspace_bytes(n_tri, P);
+        void *ws = nullptr;
+        e = hipMalloc(&ws, ws_bytes);
+        float tree_box[6];
+        if (e == hipSuccess)
+            e = bvh_build_device(o->boxes, n_tri, P, o->bvh, (int *)(o->bvh + P), tree_box, ws, ws_bytes, nullptr);
+        (void)hipFree(ws);
+        if (e != hipSuccess)
+            return fail(DMAP_ERR_HIP, "object bounds: device BVH build: %s", hipGetErrorString(e));
+        for (int a = 0; a < 3; ++a) {
+            o->bound_lo[a] = tree_box[a];
+            o->bound_hi[a] = tree_box[3 + a];
+        }
+    } else {
+        std::vector<int> idx(n_tri);
+        for (int k = 0; k < n_tri; ++k)
+            idx[k] = k;
+        std::vector<BvhBuildNode> build((size_t)P, BvhBuildNode{});
+        std::vector<int> leaf((size_t)P, -1);
+        BvhBuildNode &root = build[0];
+        build_bvh_node(build, leaf, o->boxes_host, idx, 1, 0, n_tri, (uint32_t)P, root.lo[0], root.hi[0]);
+        root.empty = 2;
+        std::vector<BvhNode> nodes((size_t)P, BvhNode{});
+        for (int i = 0; i < P; ++i) {
+            const BvhBuildNode &b = build[(size_t)i];
+            BvhNode &nd = nodes[(size_t)i];
+            nd.axis = b.axis;
+            nd.empty = b.empty;
+            for (int c = 0; c < 2; ++c)
+                if (!(b.empty & (1 << c)))
+                    bvh_centre_radius(b.lo[c], b.hi[c], nd.ctr[c], nd.rad[c]);
+        }
+        for (int a = 0; a < 3; ++a) {
+            o->bound_lo[a] = root.lo[0][a];
+            o->bound_hi[a] = root.hi[0][a];
+        }
+        e = hipMemcpy(o->bvh, nodes.data(), sizeof(BvhNode) * P, hipMemcpyHostToDevice);
+        if (e == hipSuccess)
+            e = hipMemcpy(o->bvh + P, leaf.data(), sizeof(int) * P, hipMemcpyHostToDevice);
+        if (e != hipSuccess)
+            return fail(DMAP_ERR_HIP, "object bounds: %s", hipGetErrorString(e));
     }
     o->const_normal = 1;
     for (int k = 0; k < n_tri; ++k)
         o->const_normal &= o->host[k].const_normal;
-    for (int a = 0; a < 3; ++a) {
-        o->bound_lo[a] = root.lo[0][a];
-        o->bound_hi[a] = root.hi[0][a];
-    }
     o->view.pyr = view_of(tile);
     o->view.tris = o->tris;
     o->view.boxes = o->boxes;
@@ -791,13 +853,7 @@ static int object_bounds(dmap_object *o, const std::vector<HostTri> &tris)
     o->view.bvh = o->bvh;
     o->view.bvh_leaf = (const int *)(o->bvh + P);
     o->view.bvh_leaves = P;
-    e = hipMemcpy(o->bvh, o->bvh_host.data(), sizeof(BvhNode) * P, hipMemcpyHostToDevice);
-    if (e == hipSuccess)
-        e = hipMemcpy(o->bvh + P, o->bvh_leaf_host.data(), sizeof(int) * P, hipMemcpyHostToDevice);
-    if (e == hipSuccess)
-        e = hipMemcpy(o->boxes, o->boxes_host.data(), sizeof(float) * 6 * n_tri, hipMemcpyHostToDevice);
-    if (e == hipSuccess)
-        e = hipMemcpy(o->tris, o->host.data(), sizeof(TriRecord) * n_tri, hipMemcpyHostToDevice);
+    e = hipMemcpy(o->tris, o->host.data(), sizeof(TriRecord) * n_tri, hipMemcpyHostToDevice);
     if (e != hipSuccess)
         return fail(DMAP_ERR_HIP, "object bounds: %s", hipGetErrorString(e));
     return DMAP_OK;
@@ -806,8 +862,16 @@ static int object_bounds(dmap_object *o, const std::vector<HostTri> &tris)
 int dmap_object_create(dmap_ctx *ctx, dmap_tile *tile, int n_tri, const double *positions, const double *normals,
                        const double *uvs, dmap_object **out)
 {
+    return dmap_object_create_ex(ctx, tile, n_tri, positions, normals, uvs, DMAP_BVH_AUTO, out);
+}
+
+int dmap_object_create_ex(dmap_ctx *ctx, dmap_tile *tile, int n_tri, const double *positions, const double *normals,
+                          const double *uvs, int bvh_builder, dmap_object **out)
+{
     if (!ctx || !tile || !positions || !normals || !uvs || !out || n_tri <= 0)
         return fail(DMAP_ERR_INVALID, "dmap_object_create: bad argument");
+    if (bvh_builder != DMAP_BVH_AUTO && bvh_builder != DMAP_BVH_HOST && bvh_builder != DMAP_BVH_DEVICE)
+        return fail(DMAP_ERR_INVALID, "dmap_object_create: unknown BVH builder %d", bvh_builder);
     if (n_tri > (1 << 30))
         return fail(DMAP_ERR_INVALID, "dmap_object_create: at most 2^30 triangles per object");
     if (tile->ctx->device != ctx->device)
@@ -818,14 +882,24 @@ int dmap_object_create(dmap_ctx *ctx, dmap_tile *tile, int n_tri, const double *
     o->ctx = ctx;
     o->tile = tile;
     o->n_tri = n_tri;
+    o->bvh_builder = bvh_builder;
     o->host.resize(n_tri);
+    // Per triangle: its chart and record; the first failure (lowest k) is
+    // reported.
+    std::vector<int> bad(n_tri, 0), root_levels(n_tri, -1);
+    parallel_for(n_tri, [&](int k) {
+        if (!make_tri(positions + 9 * (size_t)k, normals + 9 * (size_t)k, uvs + 6 * (size_t)k, tris[k]))
+            bad[k] = 1;
+        else if (!fill_record(tris[k], tile->n, o->host[k], &root_levels[k]))
+            bad[k] = 2;
+    });
     for (int k = 0; k < n_tri; ++k) {
-        if (!make_tri(positions + 9 * (size_t)k, normals + 9 * (size_t)k, uvs + 6 * (size_t)k, tris[k])) {
+        if (bad[k] == 1) {
             delete o;
             return fail(DMAP_ERR_INVALID, "dmap_object_create: triangle %d is degenerate in the texture plane", k);
         }
-        int root_level = -1;
-        if (!fill_record(tris[k], tile->n, o->host[k], &root_level)) {
+        if (bad[k] == 2) {
+            const int root_level = root_levels[k];
             delete o;
             if (root_level > MAX_ROOT_LEVEL)
                 return fail(DMAP_ERR_INVALID,
@@ -866,9 +940,10 @@ int dmap_object_refresh(dmap_object *obj)
     HIP_TRY(hipSetDevice(obj->ctx->device));
     HIP_TRY(hipDeviceSynchronize());
     std::vector<HostTri> tris(obj->n_tri);
-    for (int k = 0; k < obj->n_tri; ++k)
+    parallel_for(obj->n_tri, [&](int k) {
         make_tri(obj->positions.data() + 9 * (size_t)k, obj->normals.data() + 9 * (size_t)k,
                  obj->uvs.data() + 6 * (size_t)k, tris[k]);
+    });
     return object_bounds(obj, tris);
 }
 

@@ -44,9 +44,8 @@ struct dmap_object
     float bound_lo[3] = {0, 0, 0}, bound_hi[3] = {0, 0, 0};
     std::vector<TriRecord> host;
     std::vector<float> boxes_host, h_range_host;
-    std::vector<BvhNode> bvh_host;
-    std::vector<int> bvh_leaf_host;
     int bvh_leaves = 1;
+    int bvh_builder = DMAP_BVH_AUTO; // dmap_object_create_ex's choice, kept for dmap_object_refresh
     TriRecord *tris = nullptr;
     float *boxes = nullptr;
     BvhNode *bvh = nullptr; // bvh_leaves nodes, then bvh_leaves leaf entries (int)

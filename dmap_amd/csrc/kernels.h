@@ -181,6 +181,15 @@ struct TraceParams
 
 static_assert(sizeof(TraceParams) + sizeof(StepDesc) <= 4096, "a step's kernel arguments exceed 4 KB");
 
+// The top-level BVH built on the device (bvh.hip) from n triangle boxes
+// already in device memory (n x 6 fp32: lo xyz, hi xyz) into P = 2^k >= n
+// leaves: nodes (P BvhNodes) and leaf (P triangle indices, -1 for none).
+// tree_box (host memory) receives the whole tree's box. Waits for the
+// stream.
+size_t bvh_build_workspace_bytes(int n, int P);
+hipError_t bvh_build_device(const float *boxes, int n, int P, BvhNode *nodes, int *leaf, float tree_box[6],
+                            void *workspace, size_t workspace_bytes, hipStream_t stream);
+
 // Sort rays for coherence (reorder.hip): writes a permutation of 0..n-1
 // into the workspace and returns it in *perm.
 size_t reorder_workspace_bytes(int64_t n);

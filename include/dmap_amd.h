@@ -176,7 +176,18 @@ void dmap_tile_destroy(dmap_tile *tile);
 int dmap_object_create(dmap_ctx *ctx, dmap_tile *tile, int n_tri, const double *positions, const double *normals,
                        const double *uvs, dmap_object **out);
 /* The top-level BVH over the triangles' AABBs stands in for the reference's
- * embree scene (ObjectIntersector::build_bvh, displaced_intersector.cpp:540-554). */
+ * embree scene (ObjectIntersector::build_bvh, displaced_intersector.cpp:540-554).
+ * dmap_object_create_ex chooses its builder: on the host by median splits
+ * (DMAP_BVH_HOST), on the device over the Morton order of the boxes' centres
+ * (DMAP_BVH_DEVICE), or the device for 16,384 triangles and more
+ * (DMAP_BVH_AUTO, what dmap_object_create does). The trees differ, the
+ * closest hits do not (up to which of two triangles reports a hit at
+ * exactly equal t). No counterpart in the reference beyond build_bvh. */
+#define DMAP_BVH_AUTO 0
+#define DMAP_BVH_HOST 1
+#define DMAP_BVH_DEVICE 2
+int dmap_object_create_ex(dmap_ctx *ctx, dmap_tile *tile, int n_tri, const double *positions, const double *normals,
+                          const double *uvs, int bvh_builder, dmap_object **out);
 /* Per-triangle set-up: root level, root count, roots (4 x (i, j)), the
  * object-space AABB (lo xyz, hi xyz, outward-rounded fp32) and the height
  * range over the footprint. */

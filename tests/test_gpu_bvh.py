@@ -5,7 +5,8 @@ displaced_intersector.cpp:578-599), and any-hit agrees with closest-hit. The
 counts cover a single leaf, padded trees (3, 5, 127, 129, 200 triangles), a
 full tree (64, 128), and both sides of the 128-leaf limit of the workgroup
 copy in LDS (single-object launches read the tree from LDS up to 128 leaves
-and from global memory beyond)."""
+and from global memory beyond), each with the host builder's tree (median
+splits) and the device builder's (Morton order)."""
 import numpy as np
 import pytest
 
@@ -37,14 +38,15 @@ def _rays(n, lo, hi, seed):
     return r
 
 
+@pytest.mark.parametrize("bvh", ["host", "device"])
 @pytest.mark.parametrize("n_tri", [1, 3, 5, 64, 127, 128, 129, 200])
-def test_bvh_closest_and_any_hit(torch_gpu, n_tri):
+def test_bvh_closest_and_any_hit(torch_gpu, n_tri, bvh):
     import dmap_amd
     torch = torch_gpu
     tris = W.quad_plane(10, 10.0)[:n_tri]
     tex = W.c2_map(64)
     tile = dmap_amd.DisplacementTile(tex, 0.1, 0.0, True)
-    obj = dmap_amd.ObjectIntersector(tile, tris)
+    obj = dmap_amd.ObjectIntersector(tile, tris, bvh=bvh)
     xy = tris[:, 0:9].reshape(-1, 3, 3)[:, :, 0:2].reshape(-1, 2)
     rays = _rays(4096, xy.min(axis=0), xy.max(axis=0), seed=n_tri)
     g = gpu_hits(obj, rays)

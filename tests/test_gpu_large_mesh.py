@@ -1,8 +1,11 @@
-"""A base mesh of 51,200 triangles (a torus, interpolated normals, one tile
-wrapped around it eight times): object set-up time, and closest hits against
-the oracle's flat loop over every triangle (displaced_intersector.cpp:578-599)
-on a ray sample, under tests/parity.py's bar (no unexplained difference in
-hit/miss, t, st or n); the GPU's BVH stands in for that loop."""
+"""Large base meshes: tori of 51,200 and 1,048,352 triangles (interpolated
+normals, one tile wrapped around the ring eight times). Object set-up time,
+and closest hits against the oracle's flat loop over every triangle
+(displaced_intersector.cpp:578-599) on a ray sample, under tests/parity.py's
+bar (no unexplained difference in hit/miss, t, st or n); the GPU's BVH
+stands in for that loop. Both meshes get the device BVH builder (16,384
+triangles and more); on the smaller one the host builder's tree gives the
+same closest hits."""
 import time
 
 import numpy as np
@@ -17,21 +20,41 @@ pytestmark = pytest.mark.gpu
 
 def torus(nu=160, nv=160, R=1.0, r=0.35, tiles=8):
     """Rows p0 p1 p2, m0 m1 m2, t0 t1 t2 of a (nu x nv)-quad torus; the
-    texture coordinates run over `tiles` repeats of the tile around the ring."""
+    texture coordinates run over `tiles` repeats of the tile around the ring.
+    Quad (i, j) gives the triangles (00, 10, 11) and (00, 11, 01), rows in
+    the order j, i."""
     def pt(i, j):
         u, v = 2 * np.pi * i / nu, 2 * np.pi * j / nv
-        c = np.array([np.cos(u), np.sin(u), 0.0])
-        p = (R + r * np.cos(v)) * c + r * np.sin(v) * np.array([0, 0, 1.0])
-        n = np.cos(v) * c + np.sin(v) * np.array([0, 0, 1.0])
-        return p, n, np.array([tiles * i / nu, j / nv])
-    rows = []
-    for j in range(nv):
-        for i in range(nu):
-            (p00, n00, t00), (p10, n10, t10) = pt(i, j), pt(i + 1, j)
-            (p01, n01, t01), (p11, n11, t11) = pt(i, j + 1), pt(i + 1, j + 1)
-            rows.append(np.concatenate([p00, p10, p11, n00, n10, n11, t00, t10, t11]))
-            rows.append(np.concatenate([p00, p11, p01, n00, n11, n01, t00, t11, t01]))
-    return np.asarray(rows)
+        c = np.stack([np.cos(u), np.sin(u), np.zeros_like(u)], axis=-1)
+        z = np.stack([np.zeros_like(v), np.zeros_like(v), np.ones_like(v)], axis=-1)
+        p = (R + r * np.cos(v))[..., None] * c + (r * np.sin(v))[..., None] * z
+        n = np.cos(v)[..., None] * c + np.sin(v)[..., None] * z
+        return p, n, np.stack([tiles * i / nu, j / nv], axis=-1)
+    j, i = np.meshgrid(np.arange(nv, dtype=np.float64), np.arange(nu, dtype=np.float64), indexing="ij")
+    (p00, n00, t00), (p10, n10, t10) = pt(i, j), pt(i + 1, j)
+    (p01, n01, t01), (p11, n11, t11) = pt(i, j + 1), pt(i + 1, j + 1)
+    a = np.concatenate([p00, p10, p11, n00, n10, n11, t00, t10, t11], axis=-1)
+    b = np.concatenate([p00, p11, p01, n00, n11, n01, t00, t11, t01], axis=-1)
+    return np.stack([a, b], axis=2).reshape(-1, 24)
+
+
+def _sample_rays(n, seed):
+    rng = np.random.default_rng(seed)
+    o = rng.normal(size=(n, 3)) * np.array([0.4, 0.4, 0.2]) + np.array([0.0, 0.0, 2.5])
+    tgt = rng.normal(size=(n, 3)) * np.array([0.9, 0.9, 0.2])
+    return W.pack_rays(o.astype(np.float32), (tgt - o).astype(np.float32))
+
+
+def _parity(obj, tex, tris, rays, torch):
+    res = obj.intersect(torch.from_numpy(rays).cuda())
+    g = {"t": res["t"].cpu().numpy(), "prim": res["prim"].cpu().numpy(), "st": res["st"].cpu().numpy().T,
+         "ng": res["ng"].cpu().numpy().T}
+    sc = oracle.Scene(W.close_tile(tex).astype(np.float64), 0.02, 0.0, True, tris)
+    dd = rays[3:6].T.astype(np.float64)
+    ref = sc.intersect(rays[0:3].T.astype(np.float64), dd)
+    cmp = compare(g, ref, dd, tris, tex.shape[0])
+    print({k: v for k, v in cmp.items() if k not in ("bad_index", "ray_flag")}, flush=True)
+    return cmp
 
 
 def test_large_mesh_setup_and_parity(torch_gpu):
@@ -46,19 +69,51 @@ def test_large_mesh_setup_and_parity(torch_gpu):
     setup_s = time.perf_counter() - t0
     print(f"[large mesh] {len(tris)} triangles: object set-up {setup_s * 1e3:.1f} ms", flush=True)
     assert setup_s < 5.0
-    rng = np.random.default_rng(3)
     n = 4096
-    o = rng.normal(size=(n, 3)) * np.array([0.4, 0.4, 0.2]) + np.array([0.0, 0.0, 2.5])
-    tgt = rng.normal(size=(n, 3)) * np.array([0.9, 0.9, 0.2])
-    d = tgt - o
-    rays = W.pack_rays(o.astype(np.float32), d.astype(np.float32))
-    res = obj.intersect(torch.from_numpy(rays).cuda())
-    g = {"t": res["t"].cpu().numpy(), "prim": res["prim"].cpu().numpy(), "st": res["st"].cpu().numpy().T,
-         "ng": res["ng"].cpu().numpy().T}
-    sc = oracle.Scene(W.close_tile(tex).astype(np.float64), 0.02, 0.0, True, tris)
-    dd = rays[3:6].T.astype(np.float64)
-    ref = sc.intersect(rays[0:3].T.astype(np.float64), dd)
-    cmp = compare(g, ref, dd, tris, tex.shape[0])
-    print({k: v for k, v in cmp.items() if k not in ("bad_index", "ray_flag")}, flush=True)
+    cmp = _parity(obj, tex, tris, _sample_rays(n, 3), torch)
+    assert cmp["hits_ref"] > 0.3 * n
+    assert cmp["unexplained"] == 0
+
+
+def test_device_and_host_bvh_agree(torch_gpu):
+    """The same 51,200-triangle object with either builder's tree: the
+    trees differ (Morton order against median splits), the closest hits on
+    16,384 rays do not. Only a hit at exactly equal t on two triangles (a
+    shared edge) may name the other triangle."""
+    import dmap_amd
+    torch = torch_gpu
+    tris = torus()
+    tile = dmap_amd.DisplacementTile(W.c2_map(256), 0.02, 0.0, True)
+    rays = torch.from_numpy(_sample_rays(16384, 5)).cuda()
+    out = {}
+    for bvh in ("host", "device"):
+        obj = dmap_amd.ObjectIntersector(tile, tris, bvh=bvh)
+        r = obj.intersect(rays)
+        out[bvh] = {k: v.cpu().numpy() for k, v in r.items()}
+    a, b = out["host"], out["device"]
+    assert np.isfinite(a["t"]).mean() > 0.3
+    assert np.array_equal(a["t"], b["t"])
+    same = a["prim"] == b["prim"]
+    assert same.mean() > 0.999, (~same).sum()
+    for k in ("st", "bary", "ng", "texel"):
+        assert np.array_equal(a[k][..., same], b[k][..., same]), k
+
+
+def test_million_triangle_mesh(torch_gpu):
+    """1,048,352 triangles (a 724 x 724-quad torus): set-up with the
+    device BVH builder, and parity with the oracle's flat loop on 512 rays."""
+    import dmap_amd
+    torch = torch_gpu
+    tris = torus(724, 724)
+    tex = W.c2_map(256)
+    tile = dmap_amd.DisplacementTile(tex, 0.02, 0.0, True)
+    t0 = time.perf_counter()
+    obj = dmap_amd.ObjectIntersector(tile, tris)
+    torch.cuda.synchronize()
+    setup_s = time.perf_counter() - t0
+    print(f"[large mesh] {len(tris)} triangles: object set-up {setup_s * 1e3:.1f} ms", flush=True)
+    assert setup_s < 10.0
+    n = 512
+    cmp = _parity(obj, tex, tris, _sample_rays(n, 7), torch)
     assert cmp["hits_ref"] > 0.3 * n
     assert cmp["unexplained"] == 0
