@@ -438,6 +438,28 @@ int dmap_pathtrace(const dmap_object *obj, const double *cam_o, const double *ca
         (void)hipFreeAsync(ws, s);
         return fail(DMAP_ERR_MEMORY, "dmap_pathtrace: pinned counters");
     }
+    // A bounce's next-event rays are traced on a second stream while the
+    // next bounce's closest-hit batch runs on `s` (the tails of the two
+    // persistent launches overlap); their contributions are added on `s`
+    // before the next shade, in the same order as without the overlap, so
+    // the image does not change. Both streams keep the default priority.
+    hipStream_t s2 = nullptr;
+    hipEvent_t ev_rays = nullptr, ev_occluded = nullptr;
+    if (cap > 0 && (hipStreamCreateWithFlags(&s2, hipStreamNonBlocking) != hipSuccess ||
+                    hipEventCreateWithFlags(&ev_rays, hipEventDisableTiming) != hipSuccess ||
+                    hipEventCreateWithFlags(&ev_occluded, hipEventDisableTiming) != hipSuccess))
+        rc = fail(DMAP_ERR_HIP, "dmap_pathtrace: side stream");
+    int64_t pending = 0; // next-event rays traced on s2, not yet added on s
+    auto add_pending = [&](int64_t first) {
+        if (pending == 0)
+            return true;
+        if (hipStreamWaitEvent(s, ev_occluded, 0) != hipSuccess)
+            return false;
+        hipLaunchKernelGGL(k_pt_shadow, dim3(grid_of(pending)), dim3(BLOCK), 0, s, pending, W.occluded, W.contrib,
+                           W.shadow_pid, first, W.radiance);
+        pending = 0;
+        return true;
+    };
     for (int64_t first = 0; first < n_paths && rc == DMAP_OK; first += cap) {
         int64_t count = n_paths - first < cap ? n_paths - first : cap;
         hipLaunchKernelGGL(k_pt_camera, dim3(grid_of(count)), dim3(BLOCK), 0, s, first, count, spp, cam_o, cam_d,
@@ -451,6 +473,13 @@ int dmap_pathtrace(const dmap_object *obj, const double *cam_o, const double *ca
             if ((rc = dmap_intersect_closest(obj, &R, live, &H, -1, &opt, nullptr, stream)) != DMAP_OK)
                 break;
             cnt_closest += live;
+            // The previous bounce's next-event contributions, before this
+            // bounce's shade adds to the same paths and refills the
+            // next-event queue.
+            if (!add_pending(first)) {
+                rc = fail(DMAP_ERR_HIP, "dmap_pathtrace: stream wait");
+                break;
+            }
             bool last = b == depth;
             if (hipMemsetAsync(W.counters, 0, 2 * sizeof(uint32_t), s) != hipSuccess) {
                 rc = fail(DMAP_ERR_HIP, "dmap_pathtrace: counter reset");
@@ -488,20 +517,38 @@ int dmap_pathtrace(const dmap_object *obj, const double *cam_o, const double *ca
             int64_t n_next = pinned[0], n_shadow = pinned[1];
             if (n_shadow > 0) {
                 dmap_rays SR = as_rays(W.shadow);
-                if ((rc = dmap_intersect_any(obj, &SR, n_shadow, W.occluded, -1, &opt, stream)) != DMAP_OK)
+                if (hipEventRecord(ev_rays, s) != hipSuccess || hipStreamWaitEvent(s2, ev_rays, 0) != hipSuccess) {
+                    rc = fail(DMAP_ERR_HIP, "dmap_pathtrace: stream wait");
                     break;
+                }
+                if ((rc = dmap_intersect_any(obj, &SR, n_shadow, W.occluded, -1, &opt, s2)) != DMAP_OK)
+                    break;
+                if (hipEventRecord(ev_occluded, s2) != hipSuccess) {
+                    rc = fail(DMAP_ERR_HIP, "dmap_pathtrace: event");
+                    break;
+                }
                 cnt_shadow += n_shadow;
-                hipLaunchKernelGGL(k_pt_shadow, dim3(grid_of(n_shadow)), dim3(BLOCK), 0, s, n_shadow, W.occluded,
-                                   W.contrib, W.shadow_pid, first, W.radiance);
+                pending = n_shadow;
             }
             live = n_next;
             cur ^= 1;
         }
+        if (rc == DMAP_OK && !add_pending(first))
+            rc = fail(DMAP_ERR_HIP, "dmap_pathtrace: stream wait");
         if (rc == DMAP_OK)
             hipLaunchKernelGGL(k_pt_gather, dim3(grid_of(count / spp)), dim3(BLOCK), 0, s, first / spp, count / spp,
                                spp, W.radiance, image);
     }
     hipError_t e = hipGetLastError();
+    if (s2) {
+        // Work still queued on s2 (after an error) ends before its buffers go.
+        (void)hipStreamSynchronize(s2);
+        (void)hipStreamDestroy(s2);
+    }
+    if (ev_rays)
+        (void)hipEventDestroy(ev_rays);
+    if (ev_occluded)
+        (void)hipEventDestroy(ev_occluded);
     if (ws)
         (void)hipFreeAsync(ws, s);
     if (pinned) {
