@@ -55,8 +55,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-sample", type=int, default=1, help="oracle sample: every k-th ray")
     ap.add_argument("--precision", default="fp32", choices=["fp32", "fp64"])
-    ap.add_argument("--reorder", action="store_true",
-                    help="sort the rays by direction octant and origin before tracing (default: as given)")
+    ap.add_argument("--order", default="near_first", choices=["near_first", "given", "sort"],
+                    help="the order the rays are traced in: rays starting inside the object's bounds first "
+                         "(default), as given, or sorted by direction octant and origin")
     ap.add_argument("--serve-seconds", type=float, default=2.0,
                     help="length of the fixed-rate serving run (0: no serving section)")
     ap.add_argument("--serve-load", type=float, default=0.8,
@@ -90,7 +91,7 @@ def main():
     obj = dmap_amd.ObjectIntersector(tile, wl.triangles)
     rays = torch.from_numpy(wl.rays).cuda()
     hits = obj.alloc_hits(n)
-    opt = dmap_amd.TraversalOptions(precision=args.precision, reorder=args.reorder)
+    opt = dmap_amd.TraversalOptions(precision=args.precision, reorder=args.order)
     stream = torch.cuda.current_stream()
     torch.cuda.synchronize()
 

@@ -31,6 +31,7 @@ BOUND = {"box": 0, "slab": 1, "both": 2}
 LEAF = {"newton": 0, "two_triangle": 1}
 PRECISION = {"fp32": 0, "fp64": 1}
 WALK_START = {"shell": 0, "roots": 1}
+ORDER = {"given": 0, "sort": 1, "near_first": 2}  # DMAP_ORDER_*
 BUILD = {"fold": 0, "direct": 1}
 CHANNELS = ("h0", "gu", "gv", "r", "ru", "rv", "h_min", "h_max")
 STAT_NAMES = ("triangle_tests", "node_tests", "leaf_tests", "sub_squares", "newton_runs", "newton_iters",
@@ -170,10 +171,14 @@ class TraversalOptions:
     slab_max_level: int = 1
     leaf: str = "newton"
     precision: str = "fp32"
-    # Sort the rays by direction octant and origin before tracing: pays when
-    # rays of different kinds arrive interleaved; rays in generation order
-    # (pixel rows, surface points in turn) trace faster as given.
-    reorder: bool = False
+    # The order the rays are traced in (results are per ray in every order):
+    # "near_first" takes the rays whose origin lies in the object's bounding
+    # box (bounces, shadow rays: the longest walks) first, the rest after,
+    # each in its given order, so a launch's drain runs on cheap rays;
+    # "given" traces them as given; "sort" sorts them by direction octant and
+    # origin, which pays when rays of different kinds arrive shuffled.
+    # (True and False stand for "sort" and "given".)
+    reorder: object = "near_first"
     polish: bool = True  # False: report the search's own root, without the double polish
     # "shell": a constant-normal triangle's walk starts at the finest block of
     # cells holding the ray's clip to its height shell (the same hits, fewer
@@ -184,8 +189,11 @@ class TraversalOptions:
     def _c(self):
         if self.walk_start not in WALK_START:
             raise DmapError(f"walk_start {self.walk_start!r}: 'shell' or 'roots'")
+        order = {True: "sort", False: "given"}.get(self.reorder, self.reorder)
+        if order not in ORDER:
+            raise DmapError(f"reorder {self.reorder!r}: one of {sorted(ORDER)}")
         return _Options(BOUND[self.bound], int(self.slab_max_level), LEAF[self.leaf], PRECISION[self.precision],
-                        1 if self.reorder else 0, 1 if self.polish else 0, WALK_START[self.walk_start])
+                        ORDER[order], 1 if self.polish else 0, WALK_START[self.walk_start])
 
 
 class Context:

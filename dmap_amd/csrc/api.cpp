@@ -980,13 +980,14 @@ void dmap_object_destroy(dmap_object *obj)
 
 static int check_options(const dmap_options *opt, dmap_options &o)
 {
-    o = dmap_options{2, 1, 0, 0, 0, 1, 0};
+    o = dmap_options{2, 1, 0, 0, DMAP_ORDER_NEAR_FIRST, 1, 0};
     if (opt)
         o = *opt;
     if (o.bound < 0 || o.bound > 2 || o.leaf < 0 || o.leaf > 1 || o.precision < 0 || o.precision > 1 ||
-        o.walk_start < 0 || o.walk_start > 1)
-        return fail(DMAP_ERR_INVALID, "bad traversal options (bound %d, leaf %d, precision %d, walk_start %d)",
-                    o.bound, o.leaf, o.precision, o.walk_start);
+        o.walk_start < 0 || o.walk_start > 1 || o.reorder < 0 || o.reorder > 2)
+        return fail(DMAP_ERR_INVALID,
+                    "bad traversal options (bound %d, leaf %d, precision %d, reorder %d, walk_start %d)", o.bound,
+                    o.leaf, o.precision, o.reorder, o.walk_start);
     return DMAP_OK;
 }
 
@@ -1049,9 +1050,12 @@ static int trace(const dmap_object *obj, const dmap_rays *rays, int64_t n, const
     if (n > 0x7fffffffLL)
         return fail(DMAP_ERR_INVALID, "intersect: at most 2^31 - 1 rays per call");
     // Per-call workspace, stream-ordered: the ray queue head, and the
-    // reorder buffers when asked for.
-    bool reorder = o.reorder && n >= 4096;
-    size_t rbytes = reorder ? (reorder_workspace_bytes(n) + 255) / 256 * 256 : 0;
+    // buffers of the ray order when one is asked for (batches under 4,096
+    // rays are traced as given).
+    const int order = n >= 4096 ? o.reorder : DMAP_ORDER_GIVEN;
+    size_t rbytes = order == DMAP_ORDER_SORT         ? (reorder_workspace_bytes(n) + 255) / 256 * 256
+                    : order == DMAP_ORDER_NEAR_FIRST ? (near_first_workspace_bytes(n) + 255) / 256 * 256
+                                                     : 0;
     // A closest-hit call's raw records, after the queue head and the reorder
     // buffers.
     size_t raw_bytes = occluded ? 0 : sizeof(RawHit) * (size_t)n;
@@ -1066,9 +1070,12 @@ static int trace(const dmap_object *obj, const dmap_rays *rays, int64_t n, const
     p.queue = (unsigned *)ws;
     if (raw_bytes)
         p.raw = (RawHit *)((char *)ws + 256 + rbytes);
-    if (reorder) {
+    if (order != DMAP_ORDER_GIVEN) {
         uint32_t *perm = nullptr;
-        hipError_t e = reorder_rays(p.rays, n, obj->bound_lo, obj->bound_hi, (char *)ws + 256, rbytes, &perm, s);
+        hipError_t e = order == DMAP_ORDER_SORT
+                           ? reorder_rays(p.rays, n, obj->bound_lo, obj->bound_hi, (char *)ws + 256, rbytes, &perm, s)
+                           : near_first_rays(p.rays, n, obj->bound_lo, obj->bound_hi, (char *)ws + 256, rbytes, &perm,
+                                             s);
         if (e != hipSuccess) {
             if (!fixed_ws)
                 (void)hipFreeAsync(ws, s);

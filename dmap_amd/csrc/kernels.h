@@ -190,8 +190,13 @@ size_t bvh_build_workspace_bytes(int n, int P);
 hipError_t bvh_build_device(const float *boxes, int n, int P, BvhNode *nodes, int *leaf, float tree_box[6],
                             void *workspace, size_t workspace_bytes, hipStream_t stream);
 
-// Sort rays for coherence (reorder.hip): writes a permutation of 0..n-1
-// into the workspace and returns it in *perm.
+// Ray orders (reorder.hip), each a permutation of 0..n-1 written into the
+// workspace and returned in *perm: near_first_rays puts the rays whose
+// origin lies in [lo, hi] first (stable); reorder_rays sorts by direction
+// octant and origin for coherence.
+size_t near_first_workspace_bytes(int64_t n);
+hipError_t near_first_rays(const RaysDev &rays, int64_t n, const float lo[3], const float hi[3], void *workspace,
+                           size_t workspace_bytes, uint32_t **perm, hipStream_t stream);
 size_t reorder_workspace_bytes(int64_t n);
 hipError_t reorder_rays(const RaysDev &rays, int64_t n, const float lo[3], const float hi[3], void *workspace,
                         size_t workspace_bytes, uint32_t **perm, hipStream_t stream);

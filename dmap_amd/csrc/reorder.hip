@@ -1,4 +1,5 @@
-// Ray reordering before a trace: rays are sorted by a 24-bit key, the
+// Ray orders before a trace (neither changes a result, only the order of
+// work). The sort: rays are sorted by a 24-bit key, the
 // direction octant over a 21-bit Morton code of the origin quantized to
 // 128 steps per axis of the object's bounding box, so that a wave's 64
 // lanes hold rays that start close together and head the same way and walk
@@ -77,6 +78,94 @@ hipError_t reorder_rays(const RaysDev &rays, int64_t n, const float lo[3], const
                                            stream);
     *perm = idx_out;
     return e;
+}
+
+// Near-first order: the rays whose origin lies inside the object's bounding
+// box (rays that start at or near the displaced surface: bounces, shadow
+// rays) go first, the others after, each class in its given order. Rays
+// that start near the surface walk long and cost the most; a persistent
+// launch that takes them last ends on them, with its waves draining one
+// expensive ray at a time. Taking them first leaves the cheap ones (rays
+// from afar, many of which leave the object's boxes at once) for the drain:
+// on 2M pinhole primaries followed by 2M cosine bounces from the surface,
+// the trace took 2.55 ms near-first against 2.95 ms as given. One block of
+// PART_BLOCK rays counts its inside rays, a scan places the blocks, and a
+// second pass scatters each ray's index into the permutation (stable in
+// both classes).
+constexpr int PART_BLOCK = 256;
+
+__device__ inline bool origin_inside(const RaysDev &rays, int64_t r, float lx, float ly, float lz, float hx, float hy,
+                                     float hz)
+{
+    const float x = rays.ox[r], y = rays.oy[r], z = rays.oz[r];
+    return x >= lx && x <= hx && y >= ly && y <= hy && z >= lz && z <= hz;
+}
+
+__global__ void __launch_bounds__(PART_BLOCK) k_near_count(RaysDev rays, int64_t n, float lx, float ly, float lz,
+                                                           float hx, float hy, float hz, uint32_t *counts)
+{
+    const int64_t r = (int64_t)blockIdx.x * PART_BLOCK + threadIdx.x;
+    const bool in = r < n && origin_inside(rays, r, lx, ly, lz, hx, hy, hz);
+    const int c = __syncthreads_count(in);
+    if (threadIdx.x == 0)
+        counts[blockIdx.x] = (uint32_t)c;
+}
+
+__global__ void __launch_bounds__(PART_BLOCK) k_near_scatter(RaysDev rays, int64_t n, float lx, float ly, float lz,
+                                                             float hx, float hy, float hz, const uint32_t *counts,
+                                                             const uint32_t *offsets, int blocks, uint32_t *perm)
+{
+    __shared__ uint32_t wave_in[PART_BLOCK / 64];
+    const int64_t r = (int64_t)blockIdx.x * PART_BLOCK + threadIdx.x;
+    const bool in = r < n && origin_inside(rays, r, lx, ly, lz, hx, hy, hz);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const uint64_t ball = __ballot(in);
+    if (lane == 0)
+        wave_in[wave] = (uint32_t)__popcll(ball);
+    __syncthreads();
+    uint32_t before = offsets[blockIdx.x];
+    for (int w = 0; w < wave; ++w)
+        before += wave_in[w];
+    before += (uint32_t)__popcll(ball & ((1ull << lane) - 1));
+    if (r >= n)
+        return;
+    const uint32_t total_in = offsets[blocks - 1] + counts[blocks - 1];
+    const uint32_t dst = in ? before : total_in + ((uint32_t)r - before);
+    perm[dst] = (uint32_t)r;
+}
+
+size_t near_first_workspace_bytes(int64_t n)
+{
+    const int64_t blocks = (n + PART_BLOCK - 1) / PART_BLOCK;
+    size_t temp = 0;
+    (void)hipcub::DeviceScan::ExclusiveSum(nullptr, temp, (const uint32_t *)nullptr, (uint32_t *)nullptr,
+                                           (int)blocks);
+    return ((size_t)n * 4 + 255) / 256 * 256 + 2 * (((size_t)blocks * 4 + 255) / 256 * 256) + temp + 256;
+}
+
+hipError_t near_first_rays(const RaysDev &rays, int64_t n, const float lo[3], const float hi[3], void *workspace,
+                           size_t workspace_bytes, uint32_t **perm, hipStream_t stream)
+{
+    const int blocks = (int)((n + PART_BLOCK - 1) / PART_BLOCK);
+    char *at = (char *)workspace;
+    uint32_t *out = (uint32_t *)at;
+    at += ((size_t)n * 4 + 255) / 256 * 256;
+    uint32_t *counts = (uint32_t *)at;
+    at += ((size_t)blocks * 4 + 255) / 256 * 256;
+    uint32_t *offsets = (uint32_t *)at;
+    at += ((size_t)blocks * 4 + 255) / 256 * 256;
+    size_t temp_bytes = workspace_bytes - (size_t)(at - (char *)workspace);
+    hipLaunchKernelGGL(k_near_count, dim3(blocks), dim3(PART_BLOCK), 0, stream, rays, n, lo[0], lo[1], lo[2], hi[0],
+                       hi[1], hi[2], counts);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess)
+        return e;
+    if ((e = hipcub::DeviceScan::ExclusiveSum(at, temp_bytes, counts, offsets, blocks, stream)) != hipSuccess)
+        return e;
+    hipLaunchKernelGGL(k_near_scatter, dim3(blocks), dim3(PART_BLOCK), 0, stream, rays, n, lo[0], lo[1], lo[2], hi[0],
+                       hi[1], hi[2], counts, offsets, blocks, out);
+    *perm = out;
+    return hipGetLastError();
 }
 
 } // namespace dmap_amd

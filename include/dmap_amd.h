@@ -56,6 +56,11 @@ typedef struct dmap_ctx dmap_ctx;
 typedef struct dmap_tile dmap_tile;
 typedef struct dmap_object dmap_object;
 
+/* dmap_options.reorder */
+#define DMAP_ORDER_GIVEN 0
+#define DMAP_ORDER_SORT 1
+#define DMAP_ORDER_NEAR_FIRST 2
+
 /* TraversalOptions (traversal_options.h:30-38). */
 typedef struct
 {
@@ -63,10 +68,14 @@ typedef struct
     int slab_max_level; /* the slab is read at levels <= this (default 1) */
     int leaf;           /* 0 certified Newton (default), 1 two-triangle */
     int precision;      /* 0 fp32 (default), 1 the same solve in fp64 on the device */
-    int reorder;        /* 0 (default): trace the rays in their given order; 1: sort them by
-                           direction octant and origin first, which pays when rays of
-                           different kinds arrive interleaved (shuffled); results are per
-                           ray either way */
+    int reorder;        /* the order the rays are traced in (results are per ray in every
+                           order; batches under 4,096 rays go as given):
+                           DMAP_ORDER_NEAR_FIRST (default): the rays whose origin lies in
+                           the object's bounding box (bounces, shadow rays: the longest
+                           walks) first, the rest after, each in its given order;
+                           DMAP_ORDER_GIVEN: as given;
+                           DMAP_ORDER_SORT: sorted by direction octant and origin, which
+                           pays when rays of different kinds arrive shuffled */
     int polish;         /* 1 (default): the reported root is polished by two Newton steps in
                            double; 0 reports the search's own root (precision's arithmetic),
                            evaluated in double, to measure the fp32 solve's error */

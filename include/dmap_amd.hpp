@@ -51,13 +51,15 @@ struct TraversalOptions
     int slab_max_level = 1;
     LeafMode leaf = LeafMode::Newton;
     bool fp64 = false;   // the same solve in double on the device
-    bool reorder = false; // sort rays by octant and origin first (pays for interleaved kinds of rays)
+    // the ray order: DMAP_ORDER_NEAR_FIRST (rays starting inside the object's bounds first),
+    // DMAP_ORDER_GIVEN, or DMAP_ORDER_SORT (by octant and origin: pays for shuffled kinds of rays)
+    int reorder = DMAP_ORDER_NEAR_FIRST;
     bool polish = true;  // polish the reported root in double (false: the search's own root)
     bool from_roots = false; // every walk starts at the triangle's roots, as the reference's does
 
     dmap_options c() const
     {
-        return dmap_options{(int)bound, slab_max_level, (int)leaf, fp64 ? 1 : 0, reorder ? 1 : 0, polish ? 1 : 0,
+        return dmap_options{(int)bound, slab_max_level, (int)leaf, fp64 ? 1 : 0, reorder, polish ? 1 : 0,
                             from_roots ? 1 : 0};
     }
 };

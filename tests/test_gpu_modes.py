@@ -128,13 +128,17 @@ def test_fp64_device_run(torch_gpu):
 
 
 def test_reorder_does_not_change_results(torch_gpu):
+    """The three ray orders (sorted, as given, near-first) give every ray the
+    same hit, bit for bit; the near-first order puts the bounce rays (their
+    origins on the surface, inside the object's box) ahead of the camera's."""
     import dmap_amd
     wl = W.c2_incoherent(n=256, width=128, height=128, n_bounce=16384)
     obj = build(wl.texels, 0.1, 0.0, True, wl.triangles)
-    a = gpu_hits(obj, wl.rays, options=dmap_amd.TraversalOptions(reorder=True))
-    b = gpu_hits(obj, wl.rays, options=dmap_amd.TraversalOptions(reorder=False))
-    for key in ("t", "prim", "st", "ng", "texel"):
-        assert np.array_equal(a[key], b[key]), key
+    out = {order: gpu_hits(obj, wl.rays, options=dmap_amd.TraversalOptions(reorder=order))
+           for order in ("sort", "given", "near_first")}
+    for order in ("sort", "near_first"):
+        for key in ("t", "prim", "st", "ng", "texel"):
+            assert np.array_equal(out[order][key], out["given"][key]), (order, key)
 
 
 @pytest.mark.parametrize("kind", ["flat", "curved"])
