@@ -45,6 +45,7 @@ EXPORTS = ("dmap_last_error", "dmap_version", "dmap_ctx_create", "dmap_ctx_destr
            "dmap_tile_destroy", "dmap_object_create", "dmap_object_create_ex", "dmap_object_triangle_info",
            "dmap_object_destroy", "dmap_intersect_closest", "dmap_intersect_closest_noalloc", "dmap_intersect_any",
            "dmap_tile_set_texels", "dmap_object_refresh", "dmap_intersect_adjoint", "dmap_hit_record",
+           "dmap_hit_record_world",
            "dmap_kernel_timing", "dmap_kernel_timing_read", "dmap_server_create", "dmap_server_submit",
            "dmap_server_poll", "dmap_server_hits", "dmap_server_copy_hits", "dmap_server_release",
            "dmap_server_idle", "dmap_server_destroy", "dmap_pathtrace")
@@ -76,6 +77,12 @@ class _Hits(ctypes.Structure):
 class _UvMap(ctypes.Structure):
     _fields_ = [("scale_u", ctypes.c_double), ("scale_v", ctypes.c_double), ("offset_u", ctypes.c_double),
                 ("offset_v", ctypes.c_double), ("flip_v", ctypes.c_int)]
+
+
+class _RayDiffs(ctypes.Structure):
+    _fields_ = [(name, ctypes.c_void_p) for name in
+                ("rx_ox", "rx_oy", "rx_oz", "rx_dx", "rx_dy", "rx_dz", "ry_ox", "ry_oy", "ry_oz", "ry_dx", "ry_dy",
+                 "ry_dz")]
 
 
 class _Stats(ctypes.Structure):
@@ -134,6 +141,8 @@ def load_library():
         L.dmap_kernel_timing.argtypes = [c.c_int]
         L.dmap_hit_record.argtypes = [vp, c.POINTER(_Rays), c.c_int64, c.POINTER(_Hits), c.POINTER(_UvMap), c.c_int,
                                       vp, vp]
+        L.dmap_hit_record_world.argtypes = [vp, c.POINTER(_Rays), c.c_int64, c.POINTER(_Hits), c.POINTER(_UvMap),
+                                            c.c_int, vp, c.POINTER(_RayDiffs), vp, vp]
         L.dmap_kernel_timing_read.argtypes = [c.POINTER(c.c_double), c.POINTER(c.c_int64)]
         L.dmap_object_refresh.argtypes = [vp]
         L.dmap_intersect_adjoint.argtypes = [vp, c.POINTER(_Rays), c.c_int64, vp, vp, vp, vp, vp,
@@ -405,13 +414,19 @@ class ObjectIntersector:
         return image, n_closest.value, n_shadow.value
 
     def hit_record(self, rays, hits, uv_scale=(1.0, 1.0), uv_offset=(0.0, 0.0), flip_v=False, twosided=False,
-                   stream=None):
+                   stream=None, object_to_world=None, diffs=None):
         """The renderer's hit record (DisplacedGeometry::surface_point +
-        compute_intersection's frame, object space) for the result of
-        intersect(rays): dict of device tensors p (3, n), ng, dpdu, dpdv,
-        uv (2, n), t, b."""
+        compute_intersection's frame) for the result of intersect(rays):
+        dict of device tensors p (3, n), ng, dpdu, dpdv, uv (2, n), t, b.
+        Object space, or world space with object_to_world ((3, 4) array:
+        world = M x + T). diffs: the rays' world-space differentials, a
+        (12, n) fp32 device tensor (rx o xyz, rx d xyz, ry o xyz, ry d xyz),
+        adding dpdx, dpdy (3, n) and dudx, dvdx, dudy, dvdy (n,)."""
         torch = _torch()
         r, n = self._rays(rays)
+        if object_to_world is not None or diffs is not None:
+            return self._hit_record_world(r, n, hits, uv_scale, uv_offset, flip_v, twosided, stream,
+                                          object_to_world, diffs)
         h = _Hits(hits["t"].data_ptr(), hits["st"][0].data_ptr(), hits["st"][1].data_ptr(),
                   hits["bary"][0].data_ptr(), hits["bary"][1].data_ptr(), hits["ng"][0].data_ptr(),
                   hits["ng"][1].data_ptr(), hits["ng"][2].data_ptr(), hits["prim"].data_ptr(),
@@ -424,6 +439,38 @@ class ObjectIntersector:
                "dmap_hit_record")
         return {"p": rec[0:3], "ng": rec[3:6], "dpdu": rec[6:9], "dpdv": rec[9:12], "uv": rec[12:14],
                 "t": rec[14:17], "b": rec[17:20]}
+
+    def _hit_record_world(self, r, n, hits, uv_scale, uv_offset, flip_v, twosided, stream, object_to_world, diffs):
+        torch = _torch()
+        h = _Hits(hits["t"].data_ptr(), hits["st"][0].data_ptr(), hits["st"][1].data_ptr(),
+                  hits["bary"][0].data_ptr(), hits["bary"][1].data_ptr(), hits["ng"][0].data_ptr(),
+                  hits["ng"][1].data_ptr(), hits["ng"][2].data_ptr(), hits["prim"].data_ptr(),
+                  hits["texel"][0].data_ptr(), hits["texel"][1].data_ptr())
+        m = _UvMap(float(uv_scale[0]), float(uv_scale[1]), float(uv_offset[0]), float(uv_offset[1]),
+                   1 if flip_v else 0)
+        xf = None
+        if object_to_world is not None:
+            xf = np.ascontiguousarray(np.asarray(object_to_world, dtype=np.float64).reshape(3, 4))
+        dp = None
+        if diffs is not None:
+            if not (isinstance(diffs, torch.Tensor) and diffs.is_cuda and diffs.dtype == torch.float32
+                    and tuple(diffs.shape) == (12, n) and diffs.is_contiguous()):
+                raise DmapError(f"diffs must be a contiguous (12, {n}) float32 device tensor")
+            dp = _RayDiffs(*[diffs[q].data_ptr() for q in range(12)])
+        planes = 30 if diffs is not None else 20
+        rec = torch.empty((planes, n), dtype=torch.float32, device="cuda")
+        _check(_lib.dmap_hit_record_world(self._h, ctypes.byref(r), n, ctypes.byref(h), ctypes.byref(m),
+                                          1 if twosided else 0,
+                                          ctypes.c_void_p(xf.ctypes.data) if xf is not None else None,
+                                          ctypes.byref(dp) if dp is not None else None,
+                                          ctypes.c_void_p(rec.data_ptr()), _stream_ptr(stream)),
+               "dmap_hit_record_world")
+        out = {"p": rec[0:3], "ng": rec[3:6], "dpdu": rec[6:9], "dpdv": rec[9:12], "uv": rec[12:14],
+               "t": rec[14:17], "b": rec[17:20]}
+        if diffs is not None:
+            out.update({"dpdx": rec[20:23], "dpdy": rec[23:26], "dudx": rec[26], "dvdx": rec[27], "dudy": rec[28],
+                        "dvdy": rec[29]})
+        return out
 
     def refresh(self):
         """Triangle bounds and BVH from the tile's current pyramid."""

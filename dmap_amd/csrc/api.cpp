@@ -1209,8 +1209,46 @@ int dmap_intersect_adjoint(const dmap_object *obj, const dmap_rays *rays, int64_
 int dmap_hit_record(const dmap_object *obj, const dmap_rays *rays, int64_t n, const dmap_hits *hits,
                     const dmap_uv_map *uv_map, int twosided, float *record, void *stream)
 {
+    return dmap_hit_record_world(obj, rays, n, hits, uv_map, twosided, nullptr, nullptr, record, stream);
+}
+
+int dmap_hit_record_world(const dmap_object *obj, const dmap_rays *rays, int64_t n, const dmap_hits *hits,
+                          const dmap_uv_map *uv_map, int twosided, const double *object_to_world,
+                          const dmap_ray_diffs *diffs, float *record, void *stream)
+{
     if (!obj || !rays || !hits || !record)
         return fail(DMAP_ERR_INVALID, "dmap_hit_record: NULL argument");
+    Instance inst{};
+    if (object_to_world) {
+        // m^-T from the cofactors: (m^-1)^T = cof(m) / det(m).
+        const double *A = object_to_world;
+        const double m[9] = {A[0], A[1], A[2], A[4], A[5], A[6], A[8], A[9], A[10]};
+        const double c[9] = {m[4] * m[8] - m[5] * m[7], m[5] * m[6] - m[3] * m[8], m[3] * m[7] - m[4] * m[6],
+                             m[2] * m[7] - m[1] * m[8], m[0] * m[8] - m[2] * m[6], m[1] * m[6] - m[0] * m[7],
+                             m[1] * m[5] - m[2] * m[4], m[2] * m[3] - m[0] * m[5], m[0] * m[4] - m[1] * m[3]};
+        const double det = m[0] * c[0] + m[1] * c[1] + m[2] * c[2];
+        if (!(std::isfinite(det) && det != 0.0))
+            return fail(DMAP_ERR_INVALID, "dmap_hit_record_world: the object-to-world matrix is singular");
+        for (int q = 0; q < 9; ++q) {
+            inst.m[q] = m[q];
+            inst.nt[q] = c[q] / det;
+        }
+        inst.t[0] = A[3];
+        inst.t[1] = A[7];
+        inst.t[2] = A[11];
+        inst.on = 1;
+    }
+    RayDiffs rd{};
+    if (diffs) {
+        const float *pl[12] = {diffs->rx_ox, diffs->rx_oy, diffs->rx_oz, diffs->rx_dx, diffs->rx_dy, diffs->rx_dz,
+                               diffs->ry_ox, diffs->ry_oy, diffs->ry_oz, diffs->ry_dx, diffs->ry_dy, diffs->ry_dz};
+        for (int q = 0; q < 12; ++q) {
+            if (!pl[q])
+                return fail(DMAP_ERR_INVALID, "dmap_hit_record_world: a ray differential array is NULL");
+            rd.p[q] = pl[q];
+        }
+        rd.on = 1;
+    }
     if (!hits->prim || !hits->bu || !hits->bv || (twosided && (!rays->dx || !rays->dy || !rays->dz)))
         return fail(DMAP_ERR_INVALID, "dmap_hit_record: hits prim, bu, bv (and ray directions when twosided) needed");
     if (n < 0)
@@ -1227,7 +1265,7 @@ int dmap_hit_record(const dmap_object *obj, const dmap_rays *rays, int64_t n, co
     RaysDev R{rays->ox, rays->oy, rays->oz, rays->dx, rays->dy, rays->dz, rays->tmin, rays->tmax};
     HitsDev H{hits->t,  hits->s,  hits->tt,   hits->bu,      hits->bv,     hits->nx,
               hits->ny, hits->nz, hits->prim, hits->texel_i, hits->texel_j};
-    HIP_TRY(launch_hit_record(obj->view, R, H, n, uv, twosided, record, (hipStream_t)stream));
+    HIP_TRY(launch_hit_record(obj->view, R, H, n, uv, twosided, inst, rd, record, (hipStream_t)stream));
     return DMAP_OK;
 }
 

@@ -226,8 +226,23 @@ struct UvMap
     double scale_u, scale_v, offset_u, offset_v;
     int flip_v;
 };
+// The world-space part of the record: the object-to-world transform
+// (world = m x + t, row-major 3 x 3; nt = m^-T for normals) and the rays'
+// world-space differentials (origins and directions of the offset rays rx
+// and ry, 12 fp32 planes: rx o xyz, rx d xyz, ry o xyz, ry d xyz).
+struct Instance
+{
+    double m[9], t[3], nt[9];
+    int on;
+};
+struct RayDiffs
+{
+    const float *p[12];
+    int on;
+};
 hipError_t launch_hit_record(const ObjectDev &obj, const RaysDev &rays, const HitsDev &hits, int64_t n,
-                             const UvMap &uv, int twosided, float *record, hipStream_t stream);
+                             const UvMap &uv, int twosided, const Instance &inst, const RayDiffs &diffs,
+                             float *record, hipStream_t stream);
 
 // Closest hit / any hit over the object; precision 0 = fp32, 1 = fp64.
 // A closest-hit launch is followed by the finalize kernel, which turns the

@@ -2170,14 +2170,15 @@ __device__ void field_at(const PyramidView &P, double s, double t, double &h, do
 }
 
 __global__ void __launch_bounds__(256) k_hit_record(ObjectDev obj, RaysDev rays, HitsDev hits, int64_t n, UvMap uv,
-                                                    int twosided, float *rec)
+                                                    int twosided, Instance inst, RayDiffs diffs, float *rec)
 {
     const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= n)
         return;
     const int prim = hits.prim[r];
+    const int planes = diffs.on ? 30 : 20;
     if (prim < 0) {
-        for (int q = 0; q < 20; ++q)
+        for (int q = 0; q < planes; ++q)
             rec[q * n + r] = 0.f;
         return;
     }
@@ -2230,13 +2231,11 @@ __global__ void __launch_bounds__(256) k_hit_record(ObjectDev obj, RaysDev rays,
     if (uv.flip_v)
         vv = 1.0 - vv;
     const double sv = uv.flip_v ? -uv.scale_v : uv.scale_v;
-    // The geometric frame in fp32, as compute_intersection builds it.
-    float ngf[3], du[3], dv[3], pf[3];
+    // The derivatives with respect to the object's uv.
+    double dpu[3], dpv[3];
     for (int q = 0; q < 3; ++q) {
-        ngf[q] = (float)ng[q];
-        du[q] = (float)(Ss[q] * uv.scale_u);
-        dv[q] = (float)(St[q] * sv);
-        pf[q] = (float)p[q];
+        dpu[q] = Ss[q] * uv.scale_u;
+        dpv[q] = St[q] * sv;
     }
     auto fcross = [](const float *a, const float *b, float *c) {
         c[0] = a[1] * b[2] - a[2] * b[1];
@@ -2250,6 +2249,43 @@ __global__ void __launch_bounds__(256) k_hit_record(ObjectDev obj, RaysDev rays,
         a[2] /= l;
         return isfinite(a[0]) && isfinite(a[1]) && isfinite(a[2]);
     };
+    // compute_intersection's two-sided flip, decided in fp32 on the
+    // object-space ray and normal.
+    bool flip = false;
+    if (twosided) {
+        float dd[3] = {rays.dx[r], rays.dy[r], rays.dz[r]};
+        const float n0[3] = {(float)ng[0], (float)ng[1], (float)ng[2]};
+        fnormalize(dd);
+        flip = -(dd[0] * n0[0] + dd[1] * n0[1] + dd[2] * n0[2]) < 0.f;
+    }
+    if (inst.on) {
+        // To world space: points by m and t, tangents by m, the normal by
+        // m^-T (renormalized).
+        double pw[3], uw[3], vw[3], nw[3];
+        for (int q = 0; q < 3; ++q) {
+            const double *row = inst.m + 3 * q, *nrow = inst.nt + 3 * q;
+            pw[q] = (row[0] * p[0] + row[1] * p[1] + row[2] * p[2]) + inst.t[q];
+            uw[q] = row[0] * dpu[0] + row[1] * dpu[1] + row[2] * dpu[2];
+            vw[q] = row[0] * dpv[0] + row[1] * dpv[1] + row[2] * dpv[2];
+            nw[q] = nrow[0] * ng[0] + nrow[1] * ng[1] + nrow[2] * ng[2];
+        }
+        const double nwl = sqrt(nw[0] * nw[0] + nw[1] * nw[1] + nw[2] * nw[2]);
+        for (int q = 0; q < 3; ++q) {
+            p[q] = pw[q];
+            dpu[q] = uw[q];
+            dpv[q] = vw[q];
+            ng[q] = nw[q] / nwl;
+        }
+    }
+    // The geometric frame in fp32, as compute_intersection builds it (in
+    // world space when a transform is given), then the two-sided flip.
+    float ngf[3], du[3], dv[3], pf[3];
+    for (int q = 0; q < 3; ++q) {
+        ngf[q] = (float)ng[q];
+        du[q] = (float)dpu[q];
+        dv[q] = (float)dpv[q];
+        pf[q] = (float)p[q];
+    }
     float bf[3], tf[3];
     fcross(ngf, du, bf);
     if (!fnormalize(bf)) {
@@ -2259,29 +2295,76 @@ __global__ void __launch_bounds__(256) k_hit_record(ObjectDev obj, RaysDev rays,
     }
     fcross(bf, ngf, tf);
     fnormalize(tf);
-    if (twosided) {
-        float dd[3] = {rays.dx[r], rays.dy[r], rays.dz[r]};
-        fnormalize(dd);
-        if (-(dd[0] * ngf[0] + dd[1] * ngf[1] + dd[2] * ngf[2]) < 0.f)
-            for (int q = 0; q < 3; ++q) {
-                ngf[q] = -ngf[q];
-                du[q] = -du[q];
-                tf[q] = -tf[q];
-            }
-    }
+    if (flip)
+        for (int q = 0; q < 3; ++q) {
+            ngf[q] = -ngf[q];
+            du[q] = -du[q];
+            tf[q] = -tf[q];
+        }
     const float vals[20] = {pf[0],  pf[1],  pf[2], ngf[0], ngf[1], ngf[2], du[0], du[1], du[2], dv[0],
                             dv[1],  dv[2],  (float)uu, (float)vv, tf[0], tf[1], tf[2], bf[0], bf[1], bf[2]};
     for (int q = 0; q < 20; ++q)
         rec[q * n + r] = vals[q];
+    if (!diffs.on)
+        return;
+    // The uv partials from the ray differentials (the renderer's
+    // compute_uv_partials, by the published tangent-plane method of PBRT,
+    // 3rd ed., sec. 10.1.1): each offset ray meets the plane through p with
+    // normal ng at px (py); dp/dx = px - p, and (du/dx, dv/dx) solves
+    // dp/dx = du/dx dpdu + dv/dx dpdv on the two axes other than ng's
+    // largest. An offset ray parallel to the plane, or a singular system,
+    // gives zeros, as does a miss.
+    double dpx[2][3] = {{0, 0, 0}, {0, 0, 0}}, duv[2][2] = {{0, 0}, {0, 0}};
+    const double plane = ng[0] * p[0] + ng[1] * p[1] + ng[2] * p[2];
+    int a0 = 0, a1 = 1;
+    if (fabs(ng[0]) > fabs(ng[1]) && fabs(ng[0]) > fabs(ng[2])) {
+        a0 = 1;
+        a1 = 2;
+    } else if (fabs(ng[1]) > fabs(ng[2])) {
+        a0 = 0;
+        a1 = 2;
+    }
+    // dpdu as the record reports it (turned by the two-sided flip).
+    const double su = flip ? -1.0 : 1.0;
+    const double A00 = su * dpu[a0], A01 = dpv[a0], A10 = su * dpu[a1], A11 = dpv[a1];
+    const double det = A00 * A11 - A01 * A10;
+    const bool solvable = fabs(det) > 1e-12 * (fabs(A00 * A11) + fabs(A01 * A10));
+    for (int k = 0; k < 2; ++k) {
+        double o[3], d[3];
+        for (int q = 0; q < 3; ++q) {
+            o[q] = diffs.p[6 * k + q][r];
+            d[q] = diffs.p[6 * k + 3 + q][r];
+        }
+        const double nd = ng[0] * d[0] + ng[1] * d[1] + ng[2] * d[2];
+        const double tx = (plane - (ng[0] * o[0] + ng[1] * o[1] + ng[2] * o[2])) / nd;
+        if (!isfinite(tx))
+            continue;
+        for (int q = 0; q < 3; ++q)
+            dpx[k][q] = (o[q] + tx * d[q]) - p[q];
+        if (!solvable)
+            continue;
+        const double b0 = dpx[k][a0], b1 = dpx[k][a1];
+        const double x0 = (A11 * b0 - A01 * b1) / det, x1 = (A00 * b1 - A10 * b0) / det;
+        if (isfinite(x0) && isfinite(x1)) {
+            duv[k][0] = x0;
+            duv[k][1] = x1;
+        }
+    }
+    const float dvals[10] = {(float)dpx[0][0], (float)dpx[0][1], (float)dpx[0][2], (float)dpx[1][0],
+                             (float)dpx[1][1], (float)dpx[1][2], (float)duv[0][0], (float)duv[0][1],
+                             (float)duv[1][0], (float)duv[1][1]};
+    for (int q = 0; q < 10; ++q)
+        rec[(20 + q) * n + r] = dvals[q];
 }
 
 hipError_t launch_hit_record(const ObjectDev &obj, const RaysDev &rays, const HitsDev &hits, int64_t n,
-                             const UvMap &uv, int twosided, float *record, hipStream_t stream)
+                             const UvMap &uv, int twosided, const Instance &inst, const RayDiffs &diffs,
+                             float *record, hipStream_t stream)
 {
     if (n <= 0)
         return hipSuccess;
     hipLaunchKernelGGL(k_hit_record, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, obj, rays, hits, n, uv,
-                       twosided, record);
+                       twosided, inst, diffs, record);
     return hipGetLastError();
 }
 

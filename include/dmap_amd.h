@@ -16,6 +16,8 @@
  *   dmap_intersect_any    the occluded callback (displaced_geometry.cpp:79-106)
  *   dmap_hit_record       DisplacedGeometry::surface_point + compute_intersection's
  *                         frame (displaced_geometry.cpp:126-183)
+ *   dmap_hit_record_world the same, then compute_intersection's world-space end,
+ *                         transform_it + compute_uv_partials (displaced_geometry.cpp:180-183)
  * The adjoint, tile texel updates, the serving loop, the wavefront path
  * tracing call, kernel timing and the device-memory helpers have no
  * counterpart in the reference (see their declarations).
@@ -238,14 +240,40 @@ int dmap_intersect_any(const dmap_object *obj, const dmap_rays *rays, int64_t n,
  * (oriented along the base face's winding; towards -d when twosided), dpdu
  * xyz, dpdv xyz, uv, frame tangent xyz, frame bitangent xyz. uv_map (NULL:
  * identity) maps tile coordinates to the object's uv (DisplacedObject::
- * object_uv, scene_spec.cpp:382-388). The world transform and ray
- * differentials stay with the renderer. Misses get zeros. */
+ * object_uv, scene_spec.cpp:382-388), in object space
+ * (dmap_hit_record_world below adds the world transform and the ray
+ * differentials). Misses get zeros. */
 typedef struct dmap_uv_map {
     double scale_u, scale_v, offset_u, offset_v;
     int flip_v;
 } dmap_uv_map;
 int dmap_hit_record(const dmap_object *obj, const dmap_rays *rays, int64_t n, const dmap_hits *hits,
                     const dmap_uv_map *uv_map, int twosided, float *record, void *stream);
+
+/* The world-space half of compute_intersection (displaced_geometry.cpp:
+ * 180-183: transform_it and compute_uv_partials, which live in the ks
+ * renderer library, not in the reference repository) on top of
+ * dmap_hit_record. object_to_world (12 host doubles, row-major 3 x 4:
+ * world = M x + T, M invertible; NULL: identity) takes p and dpdu, dpdv to
+ * world space, ng by M^-T (renormalized), and the frame is rebuilt there as
+ * compute_intersection builds it (b = unit(ng x dpdu), t = b x ng), the
+ * two-sided flip decided on the object-space ray. With diffs (the rays'
+ * world-space differentials: the origins and directions of the offset rays
+ * rx and ry, device fp32 SoA; NULL: none) the record gains 10 planes after
+ * its 20: dp/dx xyz, dp/dy xyz, du/dx, dv/dx, du/dy, dv/dy, by the
+ * tangent-plane method of PBRT (3rd ed., sec. 10.1.1): each offset ray
+ * meets the plane through p with normal ng, and (du, dv) solves
+ * dp = du dpdu + dv dpdv on the two axes other than ng's largest (zeros
+ * where an offset ray runs parallel to the plane or the system is
+ * singular). The rays and hits are the object-space ones the intersect
+ * call used. */
+typedef struct dmap_ray_diffs {
+    const float *rx_ox, *rx_oy, *rx_oz, *rx_dx, *rx_dy, *rx_dz;
+    const float *ry_ox, *ry_oy, *ry_oz, *ry_dx, *ry_dy, *ry_dz;
+} dmap_ray_diffs;
+int dmap_hit_record_world(const dmap_object *obj, const dmap_rays *rays, int64_t n, const dmap_hits *hits,
+                          const dmap_uv_map *uv_map, int twosided, const double *object_to_world,
+                          const dmap_ray_diffs *diffs, float *record, void *stream);
 
 /* Adjoint of the closest hit (no counterpart in the reference): for the
  * loss gradients dl_dt[r] = dL/dt_r and (dl_dnx, dl_dny, dl_dnz)[r] =

@@ -161,3 +161,146 @@ def test_no_hit_behind_the_origin(torch_gpu):
             assert (hit_g == hit_r).sum() >= n - 2
         else:
             assert not hit_r.any() and not hit_g.any()
+
+
+def _rotation(seed):
+    q, _ = np.linalg.qr(np.random.default_rng(seed).normal(size=(3, 3)))
+    return q * np.sign(np.linalg.det(q))
+
+
+def _move(tris, R, T):
+    """The triangle rows with positions R p + T and vertex normals R m."""
+    out = tris.copy()
+    for v in range(3):
+        out[:, 3 * v:3 * v + 3] = tris[:, 3 * v:3 * v + 3] @ R.T + T
+        out[:, 9 + 3 * v:12 + 3 * v] = tris[:, 9 + 3 * v:12 + 3 * v] @ R.T
+    return out
+
+
+def _host(rec):
+    return {k: v.cpu().numpy() for k, v in rec.items()}
+
+
+UVMAP = dict(uv_scale=(2.0, 0.5), uv_offset=(0.1, -0.2), flip_v=True)
+
+
+@pytest.mark.parametrize("kind", ["flat", "curved", "rotated"])
+@pytest.mark.parametrize("twosided", [False, True])
+def test_world_record_of_a_rigid_instance(torch_gpu, kind, twosided):
+    """dmap_hit_record_world. With the identity it is the object-space
+    record (p, dpdu, dpdv, uv bit for bit; ng, t, b to 1e-7: ng is
+    renormalized in double). With a rotation and translation (R, T) it
+    equals the plain record of the same mesh moved to world space (R p + T,
+    R m) at the same hits (a record depends on the hit's triangle and
+    barycentrics only): p to 1e-6 of the scene's size, ng, t, b to 1e-5,
+    dpdu, dpdv to 1e-5 relative, uv to 1e-6. Tracing the moved mesh with the
+    world rays finds the same hits: hit/miss on 99.9% of the rays, the same
+    triangle on the common hits and p to 2e-6 of the scene's size (the two
+    traces see the rays rounded to fp32 in different frames)."""
+    torch = torch_gpu
+    dmap_amd, tile, obj, rays, sc, tris, o, d = _setup(kind, torch)
+    hits = obj.intersect(rays)
+    a = _host(obj.hit_record(rays, hits, twosided=twosided, **UVMAP))
+    b = _host(obj.hit_record(rays, hits, twosided=twosided, object_to_world=np.hstack([np.eye(3), np.zeros((3, 1))]),
+                             **UVMAP))
+    for k in ("p", "dpdu", "dpdv", "uv"):
+        assert np.array_equal(a[k], b[k]), k
+    for k in ("ng", "t", "b"):
+        assert np.abs(a[k] - b[k]).max() <= 1e-7, k
+    R, T = _rotation(5), np.array([0.3, -1.2, 2.0])
+    moved = dmap_amd.ObjectIntersector(tile, _move(tris, R, T))
+    o_w = (o.astype(np.float64) @ R.T + T).astype(np.float32)
+    d_w = (d.astype(np.float64) @ R.T).astype(np.float32)
+    rays_w = torch.from_numpy(W.pack_rays(o_w, d_w)).cuda()
+    o_o = ((o_w.astype(np.float64) - T) @ R).astype(np.float32)
+    d_o = (d_w.astype(np.float64) @ R).astype(np.float32)
+    rays_o = torch.from_numpy(W.pack_rays(o_o, d_o)).cuda()
+    h_i = obj.intersect(rays_o)
+    xf = np.hstack([R, T[:, None]])
+    ri = _host(obj.hit_record(rays_o, h_i, twosided=twosided, object_to_world=xf, **UVMAP))
+    # The moved mesh's record at the instance's hits.
+    rw = _host(moved.hit_record(rays_w, h_i, twosided=twosided, **UVMAP))
+    pi = h_i["prim"].cpu().numpy()
+    ok = np.flatnonzero(pi >= 0)
+    assert len(ok) > 1000
+    scale = 1 + np.abs(rw["p"][:, ok]).max()
+    assert np.abs(ri["p"][:, ok] - rw["p"][:, ok]).max() <= 1e-6 * scale
+    for k in ("ng", "t", "b"):
+        assert np.abs(ri[k][:, ok] - rw[k][:, ok]).max() <= 1e-5, k
+    for k in ("dpdu", "dpdv"):
+        err = np.abs(ri[k][:, ok] - rw[k][:, ok]).max(axis=0) / np.abs(rw[k][:, ok]).max(axis=0)
+        assert err.max() <= 1e-5, k
+    assert np.abs(ri["uv"][:, ok] - rw["uv"][:, ok]).max() <= 1e-6
+    # The moved mesh traced with the world rays.
+    h_w = moved.intersect(rays_w)
+    pw = h_w["prim"].cpu().numpy()
+    assert ((pi >= 0) == (pw >= 0)).mean() > 0.999
+    both = np.flatnonzero((pi >= 0) & (pw >= 0))
+    assert (pi[both] == pw[both]).mean() > 0.999
+    rt = _host(moved.hit_record(rays_w, h_w, twosided=twosided, **UVMAP))
+    same = both[pi[both] == pw[both]]
+    assert np.abs(ri["p"][:, same] - rt["p"][:, same]).max() <= 2e-6 * scale
+
+
+def _camera_diffs(o, d, res, fov_deg, eps):
+    """Offset rays of a pinhole camera's rays: the same origins, the
+    directions moved by eps pixel spacings along the image's x and y."""
+    fwd = d / np.linalg.norm(d, axis=1, keepdims=True)
+    right = np.cross(fwd, [0.0, 0.0, 1.0])
+    right /= np.linalg.norm(right, axis=1, keepdims=True)
+    up = np.cross(right, fwd)
+    step = 2 * np.tan(np.radians(fov_deg) / 2) / res * np.linalg.norm(d, axis=1, keepdims=True)
+    rx_d = (d + eps * step * right).astype(np.float32)
+    ry_d = (d + eps * step * up).astype(np.float32)
+    return np.concatenate([o.T, rx_d.T, o.T, ry_d.T]).astype(np.float32), rx_d, ry_d
+
+
+@pytest.mark.parametrize("instanced", [False, True])
+def test_uv_partials_against_offset_rays(torch_gpu, instanced):
+    """The uv partials on a displaced plane that is exactly flat (one
+    quad, constant texels, so the tangent plane is the surface): the offset
+    rays' own hits give (u, v) and p at px, py exactly, so du/dx, dv/dx,
+    du/dy, dv/dy equal the hit uv differences and dp/dx, dp/dy the hit
+    position differences, to fp32 rounding (1e-4 relative). Instanced: the
+    same with a rotation and translation, the differentials in world space,
+    against the world-space records."""
+    import dmap_amd
+    torch = torch_gpu
+    tex = np.full((64, 64), 0.3, dtype=np.float32)
+    tile = dmap_amd.DisplacementTile(tex, 0.05)
+    tris = W.quad_plane(1, 1.0)
+    obj = dmap_amd.ObjectIntersector(tile, tris)
+    res = 64
+    o, d = W.pinhole_rays((0.5, -0.8, 0.9), (0.5, 0.5, 0.0), 40.0, res, res)
+    o, d = o.astype(np.float64), d.astype(np.float64)
+    diffs, rx_d, ry_d = _camera_diffs(o, d, res, 40.0, 0.5)
+    R, T = (_rotation(9), np.array([1.0, 2.0, -0.5])) if instanced else (np.eye(3), np.zeros(3))
+    xf = np.hstack([R, T[:, None]]) if instanced else None
+    # Object-space rays of the camera; the differentials go in world space.
+    rays = torch.from_numpy(W.pack_rays(o.astype(np.float32), d.astype(np.float32))).cuda()
+    wd = diffs.astype(np.float64).reshape(4, 3, -1)
+    wd[0] = (R @ wd[0]) + T[:, None]
+    wd[2] = (R @ wd[2]) + T[:, None]
+    wd[1] = R @ wd[1]
+    wd[3] = R @ wd[3]
+    D = torch.from_numpy(np.ascontiguousarray(wd.reshape(12, -1).astype(np.float32))).cuda()
+    hits = obj.intersect(rays)
+    rec = _host(obj.hit_record(rays, hits, diffs=D, object_to_world=xf, **UVMAP))
+    out, offset_hit = {}, hits["prim"].cpu().numpy() >= 0
+    for name, dd in (("x", rx_d), ("y", ry_d)):
+        rr = torch.from_numpy(W.pack_rays(o.astype(np.float32), dd)).cuda()
+        hh = obj.intersect(rr)
+        offset_hit &= hh["prim"].cpu().numpy() >= 0
+        out[name] = _host(obj.hit_record(rr, hh, object_to_world=xf, **UVMAP))
+    # Rays whose offset rays hit the quad too (the plane's uv is one affine
+    # map over both triangles).
+    hit = np.flatnonzero(offset_hit)
+    assert len(hit) > 1500
+    for name, du, dv, dp in (("x", "dudx", "dvdx", "dpdx"), ("y", "dudy", "dvdy", "dpdy")):
+        duv = out[name]["uv"][:, hit] - rec["uv"][:, hit]
+        got = np.stack([rec[du][hit], rec[dv][hit]])
+        assert np.abs(got - duv).max() <= 1e-4 * np.abs(duv).max() + 2e-6, name
+        dpp = out[name]["p"][:, hit] - rec["p"][:, hit]
+        assert np.abs(rec[dp][:, hit] - dpp).max() <= 1e-4 * np.abs(dpp).max() + 2e-6, name
+    miss = np.flatnonzero(hits["prim"].cpu().numpy() < 0)
+    assert all(np.all(rec[k][..., miss] == 0) for k in ("dudx", "dvdx", "dpdx"))
