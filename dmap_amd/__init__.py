@@ -44,6 +44,7 @@ EXPORTS = ("dmap_last_error", "dmap_version", "dmap_ctx_create", "dmap_ctx_destr
            "dmap_tile_rebuild", "dmap_tile_info", "dmap_tile_read_level", "dmap_tile_read_nodes",
            "dmap_tile_destroy", "dmap_object_create", "dmap_object_create_ex", "dmap_object_triangle_info",
            "dmap_object_destroy", "dmap_intersect_closest", "dmap_intersect_closest_noalloc", "dmap_intersect_any",
+           "dmap_ray_order",
            "dmap_tile_set_texels", "dmap_object_refresh", "dmap_intersect_adjoint", "dmap_hit_record",
            "dmap_hit_record_world",
            "dmap_kernel_timing", "dmap_kernel_timing_read", "dmap_server_create", "dmap_server_submit",
@@ -137,6 +138,7 @@ def load_library():
                                              c.POINTER(_Options), c.POINTER(_Stats), vp]
         L.dmap_intersect_closest_noalloc.argtypes = L.dmap_intersect_closest.argtypes
         L.dmap_intersect_any.argtypes = [vp, c.POINTER(_Rays), c.c_int64, vp, c.c_int, c.POINTER(_Options), vp]
+        L.dmap_ray_order.argtypes = [vp, c.POINTER(_Rays), c.c_int64, c.c_int, vp, vp]
         L.dmap_tile_set_texels.argtypes = [vp, vp, c.c_int, vp]
         L.dmap_kernel_timing.argtypes = [c.c_int]
         L.dmap_hit_record.argtypes = [vp, c.POINTER(_Rays), c.c_int64, c.POINTER(_Hits), c.POINTER(_UvMap), c.c_int,
@@ -182,8 +184,10 @@ class TraversalOptions:
     precision: str = "fp32"
     # The order the rays are traced in (results are per ray in every order):
     # "near_first" takes the rays whose origin lies in the object's bounding
-    # box (bounces, shadow rays: the longest walks) first, the rest after,
-    # each in its given order, so a launch's drain runs on cheap rays;
+    # box (bounces, shadow rays: the longest walks) first, longest path
+    # through the box first (in half-octave classes of that length, each in
+    # its given order), the rest after in their given order, so a launch's
+    # drain runs on cheap rays;
     # "given" traces them as given; "sort" sorts them by direction octant and
     # origin, which pays when rays of different kinds arrive shuffled.
     # (True and False stand for "sort" and "given".)
@@ -379,6 +383,19 @@ class ObjectIntersector:
             res["record_count"] = count
             res["records"] = rec_t[:min(count, int(records))].cpu().numpy().view(NODE_RECORD).reshape(-1)
         return res
+
+    def ray_order(self, rays, order="near_first", stream=None):
+        """The order a trace of these rays under TraversalOptions(reorder=
+        order) takes them in (dmap_ray_order): an int32 device tensor of the
+        ray indices in trace order (the identity under 4,096 rays)."""
+        torch = _torch()
+        if order not in ORDER:
+            raise DmapError(f"order {order!r}: one of {sorted(ORDER)}")
+        r, n = self._rays(rays)
+        perm = torch.empty(n, dtype=torch.int32, device="cuda")
+        _check(_lib.dmap_ray_order(self._h, ctypes.byref(r), n, ORDER[order], ctypes.c_void_p(perm.data_ptr()),
+                                   _stream_ptr(stream)), "dmap_ray_order")
+        return perm
 
     def pathtrace(self, cam_o, cam_d, spp, depth, light, radiance, env, albedo, seed=0, strategy="mis",
                   options=None, stream=None):

@@ -1001,6 +1001,10 @@ static std::vector<std::pair<hipEvent_t, hipEvent_t>> g_timing_events;
 // device address of a caller-provided workspace of fixed_bytes (the
 // no-allocation entry point's scratch), used instead, with no allocation,
 // free or timing event.
+// Batches under this many rays are traced in their given order (a reorder
+// would cost more than it saves).
+constexpr int64_t REORDER_MIN_RAYS = 4096;
+
 static int trace(const dmap_object *obj, const dmap_rays *rays, int64_t n, const dmap_hits *hits, uint8_t *occluded,
                  int prim_mask, const dmap_options *options, const dmap_ray_stats *stats, void *stream,
                  void *fixed_ws = nullptr, size_t fixed_bytes = 0)
@@ -1052,7 +1056,7 @@ static int trace(const dmap_object *obj, const dmap_rays *rays, int64_t n, const
     // Per-call workspace, stream-ordered: the ray queue head, and the
     // buffers of the ray order when one is asked for (batches under 4,096
     // rays are traced as given).
-    const int order = n >= 4096 ? o.reorder : DMAP_ORDER_GIVEN;
+    const int order = n >= REORDER_MIN_RAYS ? o.reorder : DMAP_ORDER_GIVEN;
     size_t rbytes = order == DMAP_ORDER_SORT         ? (reorder_workspace_bytes(n) + 255) / 256 * 256
                     : order == DMAP_ORDER_NEAR_FIRST ? (near_first_workspace_bytes(n) + 255) / 256 * 256
                                                      : 0;
@@ -1128,6 +1132,43 @@ int dmap_kernel_timing_read(double *trace_ms, int64_t *launches)
     *trace_ms = sum;
     *launches = (int64_t)evs.size();
     return rc;
+}
+
+int dmap_ray_order(const dmap_object *obj, const dmap_rays *rays, int64_t n, int reorder, uint32_t *perm,
+                   void *stream)
+{
+    if (!obj || !rays || !perm)
+        return fail(DMAP_ERR_INVALID, "dmap_ray_order: NULL object, rays or perm");
+    if (n < 0 || n > 0x7fffffffLL)
+        return fail(DMAP_ERR_INVALID, "dmap_ray_order: %lld rays, 0 .. 2^31 - 1 per call", (long long)n);
+    if (reorder != DMAP_ORDER_GIVEN && reorder != DMAP_ORDER_SORT && reorder != DMAP_ORDER_NEAR_FIRST)
+        return fail(DMAP_ERR_INVALID, "dmap_ray_order: order %d", reorder);
+    if (n == 0)
+        return DMAP_OK;
+    if (!rays->ox || !rays->oy || !rays->oz || !rays->dx || !rays->dy || !rays->dz || !rays->tmin || !rays->tmax)
+        return fail(DMAP_ERR_INVALID, "dmap_ray_order: a ray array is NULL");
+    HIP_TRY(hipSetDevice(obj->ctx->device));
+    hipStream_t s = (hipStream_t)stream;
+    const int order = n >= REORDER_MIN_RAYS ? reorder : DMAP_ORDER_GIVEN;
+    if (order == DMAP_ORDER_GIVEN) {
+        HIP_TRY(identity_order(perm, n, s));
+        return DMAP_OK;
+    }
+    const RaysDev R{rays->ox, rays->oy, rays->oz, rays->dx, rays->dy, rays->dz, rays->tmin, rays->tmax};
+    const size_t bytes = order == DMAP_ORDER_SORT ? reorder_workspace_bytes(n) : near_first_workspace_bytes(n);
+    void *ws = nullptr;
+    HIP_TRY(hipMallocAsync(&ws, bytes, s));
+    uint32_t *out = nullptr;
+    hipError_t e = order == DMAP_ORDER_SORT ? reorder_rays(R, n, obj->bound_lo, obj->bound_hi, ws, bytes, &out, s)
+                                            : near_first_rays(R, n, obj->bound_lo, obj->bound_hi, ws, bytes, &out, s);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(perm, out, sizeof(uint32_t) * (size_t)n, hipMemcpyDeviceToDevice, s);
+    const hipError_t ef = hipFreeAsync(ws, s);
+    if (e == hipSuccess)
+        e = ef;
+    if (e != hipSuccess)
+        return fail(DMAP_ERR_HIP, "dmap_ray_order: %s", hipGetErrorString(e));
+    return DMAP_OK;
 }
 
 int dmap_intersect_closest(const dmap_object *obj, const dmap_rays *rays, int64_t n, const dmap_hits *hits,

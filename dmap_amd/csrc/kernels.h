@@ -192,8 +192,10 @@ hipError_t bvh_build_device(const float *boxes, int n, int P, BvhNode *nodes, in
 
 // Ray orders (reorder.hip), each a permutation of 0..n-1 written into the
 // workspace and returned in *perm: near_first_rays puts the rays whose
-// origin lies in [lo, hi] first (stable); reorder_rays sorts by direction
-// octant and origin for coherence.
+// origin lies in [lo, hi] first, longest path through the box first (stable
+// within each class of path length); reorder_rays sorts by direction octant
+// and origin for coherence.
+hipError_t identity_order(uint32_t *perm, int64_t n, hipStream_t stream);
 size_t near_first_workspace_bytes(int64_t n);
 hipError_t near_first_rays(const RaysDev &rays, int64_t n, const float lo[3], const float hi[3], void *workspace,
                            size_t workspace_bytes, uint32_t **perm, hipStream_t stream);

@@ -74,7 +74,9 @@ typedef struct
                            order; batches under 4,096 rays go as given):
                            DMAP_ORDER_NEAR_FIRST (default): the rays whose origin lies in
                            the object's bounding box (bounces, shadow rays: the longest
-                           walks) first, the rest after, each in its given order;
+                           walks) first, longest path through the box first (in classes
+                           of half an octave of that length), the rest after in their
+                           given order;
                            DMAP_ORDER_GIVEN: as given;
                            DMAP_ORDER_SORT: sorted by direction octant and origin, which
                            pays when rays of different kinds arrive shuffled */
@@ -228,6 +230,15 @@ int dmap_intersect_closest(const dmap_object *obj, const dmap_rays *rays, int64_
 int dmap_intersect_closest_noalloc(const dmap_object *obj, const dmap_rays *rays, int64_t n, const dmap_hits *hits,
                                    int prim_mask, const dmap_options *options, const dmap_ray_stats *stats,
                                    void *stream);
+
+/* The order a closest-hit or any-hit call of these n rays traces them in
+ * under options.reorder = reorder (DMAP_ORDER_*): perm (device memory, n
+ * uint32) receives the ray indices in trace order; batches under 4,096
+ * rays are traced as given (perm[k] = k). No counterpart in the reference,
+ * which traces one ray per call; for callers that keep per-ray data in
+ * trace order, and to check the orders. */
+int dmap_ray_order(const dmap_object *obj, const dmap_rays *rays, int64_t n, int reorder, uint32_t *perm,
+                   void *stream);
 
 /* Any hit: occluded[r] = 1 when the ray meets the surface in (tmin, tmax). */
 int dmap_intersect_any(const dmap_object *obj, const dmap_rays *rays, int64_t n, uint8_t *occluded, int prim_mask,

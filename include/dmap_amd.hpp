@@ -51,7 +51,8 @@ struct TraversalOptions
     int slab_max_level = 1;
     LeafMode leaf = LeafMode::Newton;
     bool fp64 = false;   // the same solve in double on the device
-    // the ray order: DMAP_ORDER_NEAR_FIRST (rays starting inside the object's bounds first),
+    // the ray order: DMAP_ORDER_NEAR_FIRST (rays starting inside the object's bounds first,
+    // longest path through the bounds first),
     // DMAP_ORDER_GIVEN, or DMAP_ORDER_SORT (by octant and origin: pays for shuffled kinds of rays)
     int reorder = DMAP_ORDER_NEAR_FIRST;
     bool polish = true;  // polish the reported root in double (false: the search's own root)

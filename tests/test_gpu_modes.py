@@ -141,6 +141,71 @@ def test_reorder_does_not_change_results(torch_gpu):
             assert np.array_equal(out[order][key], out["given"][key]), (order, key)
 
 
+def _near_classes(rays, lo, hi):
+    """The near-first class of each ray, restated in double: for a ray whose
+    fp32 origin lies in the box [lo, hi], floor(-2 log2(L / diagonal))
+    clamped to 0 .. 15, L its path length in the box from tmin to where it
+    leaves the box (or tmax); 15 when L / diagonal is not a positive finite
+    number; 16 for the other rays. Also whether -2 log2(L / diagonal) lies
+    within 1e-4 of a class boundary, where the device's fp32 may round to
+    the other side."""
+    o, d = rays[0:3], rays[3:6].astype(np.float64)
+    tmin, tmax = rays[6].astype(np.float64), rays[7].astype(np.float64)
+    inside = np.all((o >= lo[:, None]) & (o <= hi[:, None]), axis=0)
+    lo64, hi64, o64 = lo.astype(np.float64), hi.astype(np.float64), o.astype(np.float64)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        wall = np.where(d > 0, hi64[:, None], lo64[:, None])
+        t_axis = np.where(d != 0, (wall - o64) / d, np.inf)
+        t_exit = np.minimum(tmax, t_axis.min(axis=0))
+        x = (t_exit - tmin) * np.linalg.norm(d, axis=0) / np.linalg.norm(hi64 - lo64)
+        v = -2.0 * np.log2(x)
+    ok = (x > 0) & np.isfinite(v)
+    cls = np.where(ok, np.clip(np.floor(np.where(ok, v, 0.0)), 0, 15), 15).astype(np.int64)
+    cls = np.where(inside, cls, 16)
+    near_edge = inside & ok & (np.abs(v - np.round(v)) < 1e-4)
+    return cls, near_edge
+
+
+def test_ray_orders_are_the_stated_permutations(torch_gpu):
+    """dmap_ray_order. Near-first: a permutation that takes the rays whose
+    origin is in the object's box first, in classes of their path length in
+    the box (half an octave of the diagonal each, longest first), every class
+    in its given order, then the other rays in their given order; checked
+    against the classes restated on the host. Sort: a permutation. Given,
+    and any order under 4,096 rays: the identity."""
+    import torch
+
+    import dmap_amd
+    wl = W.c2_incoherent(n=256, width=128, height=128, n_bounce=16384)
+    obj = build(wl.texels, 0.1, 0.0, True, wl.triangles)
+    n = wl.rays.shape[1]
+    rays = torch.from_numpy(np.ascontiguousarray(wl.rays)).cuda()
+    boxes = np.stack([obj.triangle_info(k)["aabb"] for k in range(wl.triangles.shape[0])])
+    lo, hi = boxes[:, 0].min(axis=0).astype(np.float32), boxes[:, 1].max(axis=0).astype(np.float32)
+    cls, edge = _near_classes(wl.rays, lo, hi)
+    n_in = int((cls < 16).sum())
+    assert 0 < n_in < n and len(np.unique(cls[cls < 16])) >= 4, np.bincount(cls)
+
+    perm = obj.ray_order(rays, "near_first").cpu().numpy().astype(np.int64)
+    assert np.array_equal(np.sort(perm), np.arange(n))
+    assert (cls[perm[:n_in]] < 16).all() and (cls[perm[n_in:]] == 16).all()
+    assert (np.diff(perm[n_in:]) > 0).all()  # the outside rays as given
+    inner = perm[:n_in][~edge[perm[:n_in]]]  # the rays whose class the fp32 rounding cannot move
+    c = cls[inner]
+    assert (np.diff(c) >= 0).all()  # longest first
+    same = np.diff(c) == 0
+    assert (np.diff(inner)[same] > 0).all()  # stable within a class
+
+    srt = obj.ray_order(rays, "sort").cpu().numpy().astype(np.int64)
+    assert np.array_equal(np.sort(srt), np.arange(n)) and not np.array_equal(srt, np.arange(n))
+    assert np.array_equal(obj.ray_order(rays, "given").cpu().numpy(), np.arange(n))
+    small = rays[:, :4095].contiguous()
+    for order in ("near_first", "sort"):
+        assert np.array_equal(obj.ray_order(small, order).cpu().numpy(), np.arange(4095))
+    with pytest.raises(dmap_amd.DmapError):
+        obj.ray_order(rays, "shuffled")
+
+
 @pytest.mark.parametrize("kind", ["flat", "curved"])
 def test_unpolished_fp32_root_meets_the_bar(torch_gpu, kind):
     """The fp32 Newton root itself, without the two double polishing steps
