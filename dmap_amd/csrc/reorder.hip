@@ -111,10 +111,10 @@ hipError_t reorder_rays(const RaysDev &rays, int64_t n, const float lo[3], const
 // each class in its given order (2.29 against 2.46 ms on the same rays,
 // with the bounces in one class; a full sort by length was no faster, and
 // ordering the outside rays by length gained nothing).
-// One block of PART_BLOCK rays counts its rays per class, a scan over the
-// class-major counts places each class's share of each block, and a second
-// pass scatters each ray's index into the permutation (stable in every
-// class).
+// One block of PART_BLOCK rays counts its rays per class and leaves each
+// ray's class and rank in the block (2 bytes), a scan over the class-major
+// counts places each class's share of each block, and a second pass
+// scatters each ray's index into the permutation (stable in every class).
 constexpr int PART_BLOCK = 256;
 constexpr int NEAR_BUCKETS = 16;
 constexpr int NEAR_CLASSES = NEAR_BUCKETS + 1; // the inside buckets, then the outside rays
@@ -151,33 +151,13 @@ __device__ inline int near_class(const RaysDev &rays, int64_t r, int64_t n, cons
     return b <= 0.f ? 0 : b >= (float)(NEAR_BUCKETS - 1) ? NEAR_BUCKETS - 1 : (int)b;
 }
 
-// The block's rays per class, class-major: counts[c * blocks + block].
+// Each ray's class and its rank among the block's rays of that class
+// (earlier waves, then earlier lanes), as code[r] = class << 8 | rank; and
+// the block's rays per class, class-major: counts[c * blocks + block].
 __global__ void __launch_bounds__(PART_BLOCK) k_near_count(RaysDev rays, int64_t n, NearBox B, int blocks,
-                                                           uint32_t *counts)
+                                                           uint16_t *code, uint32_t *counts)
 {
-    __shared__ uint32_t per_class[NEAR_CLASSES];
-    if (threadIdx.x < NEAR_CLASSES)
-        per_class[threadIdx.x] = 0u;
-    __syncthreads();
-    const int64_t r = (int64_t)blockIdx.x * PART_BLOCK + threadIdx.x;
-    const int c = near_class(rays, r, n, B);
-    const int lane = threadIdx.x & 63;
-    for (int k = 0; k < NEAR_CLASSES; ++k) {
-        const uint64_t ball = __ballot(c == k && r < n);
-        if (lane == 0 && ball)
-            atomicAdd(&per_class[k], (uint32_t)__popcll(ball));
-    }
-    __syncthreads();
-    if (threadIdx.x < NEAR_CLASSES)
-        counts[(int64_t)threadIdx.x * blocks + blockIdx.x] = per_class[threadIdx.x];
-}
-
-// Each ray's place: its class's offset for the block (the exclusive scan
-// of the class-major counts), then the rays of its class before it in the
-// block (earlier waves, then earlier lanes).
-__global__ void __launch_bounds__(PART_BLOCK) k_near_scatter(RaysDev rays, int64_t n, NearBox B, int blocks,
-                                                             const uint32_t *offsets, uint32_t *perm)
-{
+    static_assert(PART_BLOCK <= 256 && NEAR_CLASSES <= 255, "class and rank share 16 bits");
     __shared__ uint32_t wave_count[NEAR_CLASSES][PART_BLOCK / 64];
     const int64_t r = (int64_t)blockIdx.x * PART_BLOCK + threadIdx.x;
     const int c = near_class(rays, r, n, B);
@@ -191,13 +171,30 @@ __global__ void __launch_bounds__(PART_BLOCK) k_near_scatter(RaysDev rays, int64
             mine = ball;
     }
     __syncthreads();
+    if (threadIdx.x < NEAR_CLASSES) {
+        uint32_t total = 0;
+        for (int w = 0; w < PART_BLOCK / 64; ++w)
+            total += wave_count[threadIdx.x][w];
+        counts[(int64_t)threadIdx.x * blocks + blockIdx.x] = total;
+    }
     if (r >= n)
         return;
-    uint32_t dst = offsets[(int64_t)c * blocks + blockIdx.x];
+    uint32_t rank = (uint32_t)__popcll(mine & ((1ull << lane) - 1));
     for (int w = 0; w < wave; ++w)
-        dst += wave_count[c][w];
-    dst += (uint32_t)__popcll(mine & ((1ull << lane) - 1));
-    perm[dst] = (uint32_t)r;
+        rank += wave_count[c][w];
+    code[r] = (uint16_t)((uint32_t)c << 8 | rank);
+}
+
+// Each ray's place: its class's offset for the block (the exclusive scan
+// of the class-major counts) plus its rank in the block.
+__global__ void __launch_bounds__(PART_BLOCK) k_near_scatter(int64_t n, int blocks, const uint16_t *code,
+                                                             const uint32_t *offsets, uint32_t *perm)
+{
+    const int64_t r = (int64_t)blockIdx.x * PART_BLOCK + threadIdx.x;
+    if (r >= n)
+        return;
+    const uint32_t k = code[r];
+    perm[offsets[(int64_t)(k >> 8) * blocks + blockIdx.x] + (k & 255u)] = (uint32_t)r;
 }
 
 static size_t round256(size_t b) { return (b + 255) / 256 * 256; }
@@ -208,7 +205,7 @@ size_t near_first_workspace_bytes(int64_t n)
     size_t temp = 0;
     (void)hipcub::DeviceScan::ExclusiveSum(nullptr, temp, (const uint32_t *)nullptr, (uint32_t *)nullptr,
                                            (int)cells);
-    return round256((size_t)n * 4) + 2 * round256((size_t)cells * 4) + temp + 256;
+    return round256((size_t)n * 4) + round256((size_t)n * 2) + 2 * round256((size_t)cells * 4) + temp + 256;
 }
 
 hipError_t near_first_rays(const RaysDev &rays, int64_t n, const float lo[3], const float hi[3], void *workspace,
@@ -227,18 +224,20 @@ hipError_t near_first_rays(const RaysDev &rays, int64_t n, const float lo[3], co
     char *at = (char *)workspace;
     uint32_t *out = (uint32_t *)at;
     at += round256((size_t)n * 4);
+    uint16_t *code = (uint16_t *)at;
+    at += round256((size_t)n * 2);
     uint32_t *counts = (uint32_t *)at;
     at += round256((size_t)cells * 4);
     uint32_t *offsets = (uint32_t *)at;
     at += round256((size_t)cells * 4);
     size_t temp_bytes = workspace_bytes - (size_t)(at - (char *)workspace);
-    hipLaunchKernelGGL(k_near_count, dim3(blocks), dim3(PART_BLOCK), 0, stream, rays, n, B, blocks, counts);
+    hipLaunchKernelGGL(k_near_count, dim3(blocks), dim3(PART_BLOCK), 0, stream, rays, n, B, blocks, code, counts);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess)
         return e;
     if ((e = hipcub::DeviceScan::ExclusiveSum(at, temp_bytes, counts, offsets, cells, stream)) != hipSuccess)
         return e;
-    hipLaunchKernelGGL(k_near_scatter, dim3(blocks), dim3(PART_BLOCK), 0, stream, rays, n, B, blocks, offsets, out);
+    hipLaunchKernelGGL(k_near_scatter, dim3(blocks), dim3(PART_BLOCK), 0, stream, n, blocks, code, offsets, out);
     *perm = out;
     return hipGetLastError();
 }
