@@ -138,16 +138,18 @@ __device__ inline int near_class(const RaysDev &rays, int64_t r, int64_t n, cons
     if (!(o[0] >= B.lo[0] && o[0] <= B.hi[0] && o[1] >= B.lo[1] && o[1] <= B.hi[1] && o[2] >= B.lo[2] &&
           o[2] <= B.hi[2]))
         return NEAR_BUCKETS;
+    // (The class only orders work: the hardware reciprocal, square root and
+    // log2, each within a few ulps, are exact enough.)
     const float d[3] = {rays.dx[r], rays.dy[r], rays.dz[r]};
     float t_exit = rays.tmax[r];
     for (int a = 0; a < 3; ++a)
         if (d[a] != 0.f)
-            t_exit = fminf(t_exit, ((d[a] > 0.f ? B.hi[a] : B.lo[a]) - o[a]) / d[a]);
-    const float len = (t_exit - rays.tmin[r]) * sqrtf(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+            t_exit = fminf(t_exit, ((d[a] > 0.f ? B.hi[a] : B.lo[a]) - o[a]) * __builtin_amdgcn_rcpf(d[a]));
+    const float len = (t_exit - rays.tmin[r]) * __builtin_amdgcn_sqrtf(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
     const float x = len * B.inv_diag;
     if (!(x > 0.f) || !(x < INFINITY))
         return NEAR_BUCKETS - 1;
-    const float b = floorf(-2.f * log2f(x));
+    const float b = floorf(-2.f * __builtin_amdgcn_logf(x));
     return b <= 0.f ? 0 : b >= (float)(NEAR_BUCKETS - 1) ? NEAR_BUCKETS - 1 : (int)b;
 }
 
@@ -159,16 +161,24 @@ __global__ void __launch_bounds__(PART_BLOCK) k_near_count(RaysDev rays, int64_t
 {
     static_assert(PART_BLOCK <= 256 && NEAR_CLASSES <= 255, "class and rank share 16 bits");
     __shared__ uint32_t wave_count[NEAR_CLASSES][PART_BLOCK / 64];
+    for (int q = threadIdx.x; q < NEAR_CLASSES * (PART_BLOCK / 64); q += PART_BLOCK)
+        (&wave_count[0][0])[q] = 0u;
+    __syncthreads();
     const int64_t r = (int64_t)blockIdx.x * PART_BLOCK + threadIdx.x;
-    const int c = near_class(rays, r, n, B);
+    const int c = r < n ? near_class(rays, r, n, B) : -1;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    uint64_t mine = 0;
-    for (int k = 0; k < NEAR_CLASSES; ++k) {
-        const uint64_t ball = __ballot(c == k && r < n);
-        if (lane == 0)
-            wave_count[k][wave] = (uint32_t)__popcll(ball);
+    // One ballot per class present in the wave (a wave of rays from afar
+    // holds one), led by the first lane not yet counted.
+    uint64_t mine = 0, todo = __ballot(r < n);
+    while (todo) {
+        const int lead = __ffsll((unsigned long long)todo) - 1;
+        const int k = __shfl(c, lead);
+        const uint64_t ball = __ballot(c == k);
         if (c == k)
             mine = ball;
+        if (lane == lead)
+            wave_count[k][wave] = (uint32_t)__popcll(ball);
+        todo &= ~ball;
     }
     __syncthreads();
     if (threadIdx.x < NEAR_CLASSES) {
