@@ -35,7 +35,7 @@ BYTES_LEAF = 16
 # FETCH_SIZE and WRITE_SIZE on this workload, each scaled by the same
 # counter's reading for a device copy of known size (bench.py
 # --calibrate-copy); profiles/k_trace_c2_incoherent_traffic.md.
-TRAFFIC_BYTES_PER_RAY = 512.9
+TRAFFIC_BYTES_PER_RAY = 591.8
 # fp32 operations per unit of work, counted from the restated formulas
 # (node test with box + slab, sub-square bound, Newton iteration).
 FLOPS_NODE = 200
