@@ -413,7 +413,9 @@ int dmap_pathtrace(const dmap_object *obj, const double *cam_o, const double *ca
     // The queues stay in pixel order (camera rays by pixel, later bounces
     // compacted wave by wave in that order), which is coherent already:
     // sorting each batch cost more than it saved (20.4 against 17.2 ms for
-    // 1024^2 x 4 spp on the 4096^2 plane), so the calls below do not.
+    // 1024^2 x 4 spp on the 4096^2 plane), and so did the near-first order,
+    // whose length classes scatter neighbouring pixels' bounces (12.2-12.4
+    // against 11.6 ms, the same image), so the calls below trace as given.
     dmap_options opt{2, 1, 0, 0, 0, 1, 0};
     if (options)
         opt = *options;
