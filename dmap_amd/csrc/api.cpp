@@ -502,14 +502,44 @@ int dmap_ctx_create(int device, dmap_ctx **out)
     hipError_t e = hipGetSymbolAddress(&sp, HIP_SYMBOL(scratch));
     if (e != hipSuccess || !sp)
         return fail(DMAP_ERR_HIP, "dmap_ctx_create: scratch address on device %d: %s", device, hipGetErrorString(e));
+    // The workspace pool: device memory of this device, kept once used
+    // (the default pool returns it at each synchronization, and a call
+    // after a synchronization then maps it again: 2.54 against 2.49 ms per
+    // 4M-ray call, and about 0.2 ms per path-traced frame, which
+    // synchronizes once per bounce).
+    hipMemPoolProps props;
+    std::memset(&props, 0, sizeof props);
+    props.allocType = hipMemAllocationTypePinned;
+    props.location.type = hipMemLocationTypeDevice;
+    props.location.id = device;
+    hipMemPool_t pool = nullptr;
+    if ((e = hipMemPoolCreate(&pool, &props)) != hipSuccess)
+        return fail(DMAP_ERR_HIP, "dmap_ctx_create: workspace pool on device %d: %s", device, hipGetErrorString(e));
+    uint64_t keep = UINT64_MAX;
+    if ((e = hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep)) != hipSuccess) {
+        (void)hipMemPoolDestroy(pool);
+        return fail(DMAP_ERR_HIP, "dmap_ctx_create: workspace pool threshold: %s", hipGetErrorString(e));
+    }
     dmap_ctx *c = new dmap_ctx;
     c->device = device;
     c->scratch = sp;
+    c->pool = pool;
     *out = c;
     return DMAP_OK;
 }
 
-void dmap_ctx_destroy(dmap_ctx *ctx) { delete ctx; }
+void dmap_ctx_destroy(dmap_ctx *ctx)
+{
+    if (!ctx)
+        return;
+    if (ctx->pool && hipSetDevice(ctx->device) == hipSuccess) {
+        // Workspaces freed on a stream go back to the pool when that work
+        // ends: wait for it before the pool goes.
+        (void)hipDeviceSynchronize();
+        (void)hipMemPoolDestroy(ctx->pool);
+    }
+    delete ctx;
+}
 
 int dmap_device_alloc(dmap_ctx *ctx, int64_t bytes, void **ptr)
 {
@@ -1069,7 +1099,7 @@ static int trace(const dmap_object *obj, const dmap_rays *rays, int64_t n, const
             return fail(DMAP_ERR_INVALID, "intersect: %lld rays need %zu B of scratch, more than its %zu B",
                         (long long)n, 256 + rbytes + raw_bytes, fixed_bytes);
     } else {
-        HIP_TRY(hipMallocAsync(&ws, 256 + rbytes + raw_bytes, s));
+        HIP_TRY(hipMallocFromPoolAsync(&ws, 256 + rbytes + raw_bytes, obj->ctx->pool, s));
     }
     p.queue = (unsigned *)ws;
     if (raw_bytes)
@@ -1157,7 +1187,7 @@ int dmap_ray_order(const dmap_object *obj, const dmap_rays *rays, int64_t n, int
     const RaysDev R{rays->ox, rays->oy, rays->oz, rays->dx, rays->dy, rays->dz, rays->tmin, rays->tmax};
     const size_t bytes = order == DMAP_ORDER_SORT ? reorder_workspace_bytes(n) : near_first_workspace_bytes(n);
     void *ws = nullptr;
-    HIP_TRY(hipMallocAsync(&ws, bytes, s));
+    HIP_TRY(hipMallocFromPoolAsync(&ws, bytes, obj->ctx->pool, s));
     uint32_t *out = nullptr;
     hipError_t e = order == DMAP_ORDER_SORT ? reorder_rays(R, n, obj->bound_lo, obj->bound_hi, ws, bytes, &out, s)
                                             : near_first_rays(R, n, obj->bound_lo, obj->bound_hi, ws, bytes, &out, s);
@@ -1222,7 +1252,7 @@ int dmap_intersect_adjoint(const dmap_object *obj, const dmap_rays *rays, int64_
     hipStream_t s = (hipStream_t)stream;
     // The closest hits are recomputed, not kept from the forward pass.
     void *ws = nullptr;
-    HIP_TRY(hipMallocAsync(&ws, 44 * (size_t)n, s));
+    HIP_TRY(hipMallocFromPoolAsync(&ws, 44 * (size_t)n, obj->ctx->pool, s));
     float *f = (float *)ws;
     int32_t *ip = (int32_t *)(f + 8 * n);
     dmap_hits h{f, f + n, f + 2 * n, f + 3 * n, f + 4 * n, f + 5 * n, f + 6 * n, f + 7 * n, ip, ip + n, ip + 2 * n};

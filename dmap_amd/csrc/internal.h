@@ -18,6 +18,11 @@ struct dmap_ctx
     // The device address of the no-allocation entry point's scratch array
     // on this context's device (hipGetSymbolAddress at creation).
     void *scratch = nullptr;
+    // The calls' stream-ordered workspaces come from this pool, which keeps
+    // the memory it has handed out (release threshold at its maximum)
+    // instead of returning it at every synchronization, until the context
+    // is destroyed.
+    hipMemPool_t pool = nullptr;
 };
 
 struct dmap_tile

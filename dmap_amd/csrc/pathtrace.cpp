@@ -431,7 +431,7 @@ int dmap_pathtrace(const dmap_object *obj, const double *cam_o, const double *ca
     char *ws = nullptr;
     Workspace W;
     if (cap > 0)
-        HIP_TRY(hipMallocAsync((void **)&ws, W.carve(nullptr, cap), s));
+        HIP_TRY(hipMallocFromPoolAsync((void **)&ws, W.carve(nullptr, cap), obj->ctx->pool, s));
     if (ws)
         W.carve(ws, cap);
     uint32_t *pinned = nullptr;

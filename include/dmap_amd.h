@@ -146,7 +146,10 @@ typedef struct
 const char *dmap_last_error(void);
 const char *dmap_version(void);
 
-/* One HIP device. */
+/* One HIP device. The calls' per-call workspaces come from a memory pool of
+ * the context's own, which keeps the device memory it has handed out (up to
+ * the calls' largest simultaneous use) instead of returning it at every
+ * synchronization; dmap_ctx_destroy waits for the device and returns it. */
 int dmap_ctx_create(int device, dmap_ctx **out);
 void dmap_ctx_destroy(dmap_ctx *ctx);
 
