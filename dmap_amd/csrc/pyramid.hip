@@ -25,19 +25,25 @@ struct DNode
     double h0, gu, gv, r, ru, rv, hmin, hmax;
 };
 
+// The next float above f (not NaN, below +inf), by one step of its bit
+// pattern: magnitudes are ordered as unsigned integers, so a positive f
+// steps up, a negative one steps towards zero, and either zero goes to the
+// smallest positive denormal: nextafterf(f, +inf), in five integer
+// operations and no branch.
+__device__ inline float next_up_f(float f)
+{
+    const uint32_t b = __float_as_uint(f);
+    return __uint_as_float((b & 0x7fffffffu) == 0 ? 1u : (int32_t)b >= 0 ? b + 1 : b - 1);
+}
 __device__ inline float round_down_f(double x)
 {
-    float f = (float)x;
-    if ((double)f > x)
-        f = nextafterf(f, -INFINITY);
-    return f;
+    const float f = (float)x;
+    return (double)f > x ? -next_up_f(-f) : f;
 }
 __device__ inline float round_up_f(double x)
 {
-    float f = (float)x;
-    if ((double)f < x)
-        f = nextafterf(f, INFINITY);
-    return f;
+    const float f = (float)x;
+    return (double)f < x ? next_up_f(f) : f;
 }
 
 // The fp32 store of one node: the outward rounding of layout.h. `half` is
