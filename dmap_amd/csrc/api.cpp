@@ -614,7 +614,7 @@ static int tile_build(dmap_tile *t, hipStream_t stream)
 static int read_root(dmap_tile *t, hipStream_t stream)
 {
     float2 root;
-    HIP_TRY(hipMemcpyAsync(&root, t->mm, sizeof root, hipMemcpyDeviceToHost, stream));
+    HIP_TRY(hipMemcpyAsync(&root, t->mm + level_base(1), sizeof root, hipMemcpyDeviceToHost, stream));
     HIP_TRY(hipStreamSynchronize(stream));
     t->root_hmin = root.x;
     t->root_hmax = root.y;
@@ -644,7 +644,7 @@ int dmap_tile_create(dmap_ctx *ctx, const float *texels, int n, int texels_on_de
     t->build = build;
     t->strength = strength;
     t->offset = offset;
-    t->cells = ((size_t)4 * n * n - 1) / 3;
+    t->cells = pyramid_cells((uint32_t)n);
     size_t nn = (size_t)n * n, nodes = (size_t)(n + 1) * (n + 1);
     size_t scratch = fold_scratch_doubles(n);
     hipError_t e;

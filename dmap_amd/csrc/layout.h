@@ -2,6 +2,7 @@
 // records the traversal reads. Plain structs, no constructors: they are
 // copied to the device as bytes.
 #pragma once
+#include <stddef.h>
 #include <stdint.h>
 
 namespace dmap_amd
@@ -56,7 +57,16 @@ __host__ __device__ inline uint32_t compact1by1(uint32_t x)
     x = (x | (x >> 8)) & 0x0000ffffu;
     return x;
 }
-__host__ __device__ inline uint32_t level_base(uint32_t m) { return (m * m - 1u) / 3u; }
+// First entry of the level of m x m cells. Levels follow each other root
+// first, after LEVEL_PAD unused entries: (m^2 - 1) / 3 = 1 + 4 + ... + m^2 / 4
+// is 1 mod 4 below the root, so the pad puts every level, and with it every
+// Morton group of four children, on a multiple of four entries; a group is
+// then one aligned 32-byte min-max sector and one aligned 64-byte Taylor
+// line, which the walk reads as whole vectors.
+constexpr uint32_t LEVEL_PAD = 3;
+__host__ __device__ inline uint32_t level_base(uint32_t m) { return (m * m - 1u) / 3u + LEVEL_PAD; }
+// Entries of a pyramid over n x n leaves, the pad included.
+__host__ __device__ inline size_t pyramid_cells(uint32_t n) { return ((size_t)4 * n * n - 1) / 3 + LEVEL_PAD; }
 
 // What the traversal needs of one base triangle. Built on the host in
 // double from the reference's BaseTriangle (displaced_surface.cpp:54-91),
